@@ -1,0 +1,119 @@
+// cnv_device.h -- structures shared by the gfx950 kernels and the host side of libcnv.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cnv {
+
+constexpr int kStencilWidth = 7;  // every Diff1/Diff2 row fits a 7-column window
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+
+// Device-side control block of one Poisson solve.  Written only by the last workgroup of
+// each SOR launch (and by the init kernel); read by every workgroup at launch start.
+enum SorStatus : int { SOR_RUNNING = 0, SOR_CONVERGED = 1, SOR_FAILED = 2, SOR_ROLLBACK = 3 };
+
+struct SorCtl {
+    int k;              // sweeps accepted so far
+    int parity;         // which psi buffer holds the current iterate
+    int status;         // SorStatus
+    int s_rerun;        // after a rollback: sweeps to redo from the unchanged source buffer
+    unsigned ticket;    // last-workgroup election counter, reset by the last workgroup
+    int launches;       // launches that did work (diagnostic)
+    int rel_first;      // 1: threshold becomes rtol * (first sweep's update), then 0
+    int itmax;          // sweep budget of this solve
+    double e_last;      // L1 update of the last accepted sweep
+    double tol;         // current stopping threshold on the L1 update
+    double rtol;        // relative inner tolerance (defect correction)
+    double tol_floor;   // lower bound on the threshold (defect correction)
+};
+
+// Continuity diagnostic: max/min as order-preserving unsigned keys (exact, order independent).
+struct Diag {
+    unsigned long long kmax;
+    unsigned long long kmin;
+    double outer_norm;  // defect correction: L1 norm of the FP64 Jacobi-scaled residual
+    double pad;
+};
+
+__host__ __device__ inline unsigned long long dkey(double x) {
+    unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double dkey_inv(unsigned long long k) {
+    unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __builtin_bit_cast(double, b);
+}
+
+// Per-row stencil tables: row r of an operator has its nonzeros inside columns
+// [start[r], start[r] + 7); coef[r*7 + m] multiplies column start[r] + m (zero-padded).
+struct Stencils {
+    const int* start;
+    const double* d1x;  // Diff1(n, order, dx)
+    const double* d1y;  // Diff1(n, order, dy)
+    const double* d2x;  // Diff2(n, order, dx)
+    const double* d2y;  // Diff2(n, order, dy)
+    const float* f1x;   // the same tables rounded to FP32 (defect-correction transport)
+    const float* f1y;
+    const float* f2x;
+    const float* f2y;
+};
+
+struct WallBC {
+    double u1, u2, u3, u4, v1, v2, v3, v4;
+};
+
+// SOR launch arguments (one instance per precision).
+template <typename T>
+struct SorArgs {
+    const T* src_f;      // source term: w (plain path, negate = 1) or the FP32 residual
+    T* psi0;
+    T* psi1;
+    double* partial;     // [S][ntiles] per-workgroup L1 updates
+    SorCtl* ctl;
+    int nr, nc, ld, tiles_i, tiles_j;  // rows (i, y) and columns (j, x) of the grid
+    int negate;          // 1: f = -src (the reference's invsig before the solve)
+    T c1, c2, c3, B, beta, omb;  // dy*dy, dx*dx, dx*dx*dy*dy, 2*(dx*dx+dy*dy), beta, 1-beta
+};
+
+// ---- host launch wrappers (defined in the .hip files) ---------------------------------
+struct TileShape { int ti, tj, s; };
+
+int sor_fp64_launch(const SorArgs<double>& a, TileShape shape, bool gs, hipStream_t st);
+int sor_fp32_launch(const SorArgs<float>& a, TileShape shape, bool gs, hipStream_t st);
+bool sor_shape_supported(TileShape shape);
+
+int wall_vorticity_launch(double* w, const double* u, const double* v, int n, int ld,
+                          const Stencils& s, const WallBC& bc, hipStream_t st);
+int transport_fp64_launch(const double* w_in, double* w_out, const double* u, const double* v,
+                          int n, int ld, const Stencils& s, const WallBC& bc, double rRe,
+                          double dt, hipStream_t st);
+int transport_fp32_launch(const double* w_in, double* w_out, const double* u, const double* v,
+                          int n, int ld, const Stencils& s, const WallBC& bc, double rRe,
+                          double dt, hipStream_t st);
+int poisson_init_launch(double* psi0, SorCtl* ctl, Diag* diag, int n, int ld, double tol,
+                        int itmax, hipStream_t st);
+int velocity_launch(const double* psi0, const double* psi1, const SorCtl* ctl, double* u,
+                    double* v, int n, int ld, const Stencils& s, hipStream_t st);
+int continuity_launch(const double* u, const double* v, Diag* diag, int n, int ld,
+                      const Stencils& s, hipStream_t st);
+int euler_launch(double* w, const double* dwdx, const double* dwdy, const double* d2x,
+                 const double* d2y, const double* u, const double* v, double rRe, double dt,
+                 long long count, hipStream_t st);
+int add_launch(const double* a, const double* b, double* out, long long count, int sub,
+               hipStream_t st);
+int copy_launch(double* dst, const double* src, long long count, hipStream_t st);
+
+// defect correction
+int dc_residual_launch(const double* w, const double* psi, float* r32, Diag* diag, int n,
+                       int ld, double c1, double c2, double c3, double B, double beta,
+                       hipStream_t st);
+int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCtl* ctl, int n,
+                      int ld, hipStream_t st);
+int dc_inner_init_launch(float* e0, SorCtl* ctl, int n, int ld, double rtol, double floor_,
+                         int itmax, hipStream_t st);
+int zero_f32_launch(float* p, long long count, hipStream_t st);
+int velocity_fp32_launch(const double* psi, double* u, double* v, int n, int ld, const Stencils& s,
+                         hipStream_t st);
+
+}  // namespace cnv

@@ -1,0 +1,625 @@
+// context.hip -- the device-resident C ABI (include/cnv.h): contexts, state, the time step loop.
+//
+// One context = one cavity on one GPU.  The step (src/main.c:276-451) is a fixed sequence of
+// stream-ordered launches; only the Poisson solve has a data-dependent length:
+//   wall vorticity (edges of w, in place)           src/main.c:283-320
+//   transport: derivatives + explicit Euler, w -> w' (ping-pong)   src/main.c:323-344
+//   Poisson: init (psi = 0, control block reset), then SOR launches queued in chunks; each
+//            launch checks the device control block and turns into a no-op once the solve
+//            has converged, so the host only polls a pinned copy of that block between chunks
+//            and stops queueing                   src/main.c:347-363, src/poisson.c:224-285
+//   velocity: u = DY psi, v = -(DX psi)            src/main.c:366-383
+//   continuity max/min (optional diagnostic)       src/main.c:387-408
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "cnv.h"
+#include "cnv_device.h"
+#include "../host/cnv_host.h"
+
+using namespace cnv;
+
+namespace {
+
+constexpr int kSlots = 4;  // pinned copies of the control block in flight
+
+char g_err[256] = "";
+
+}  // namespace
+
+struct cnv_ctx {
+    cnv_params p;
+    int n, ld;
+    size_t cells;
+    hipStream_t stream = nullptr;
+    // state
+    double* w[2] = {nullptr, nullptr};
+    int wcur = 0;
+    double* psi[2] = {nullptr, nullptr};
+    double* u = nullptr;
+    double* v = nullptr;
+    // tables
+    int* d_start = nullptr;
+    double* d_tab = nullptr;  // 4 operators x n x 7
+    float* d_ftab = nullptr;
+    Stencils st{};
+    // solver control
+    SorCtl* ctl = nullptr;
+    Diag* diag = nullptr;
+    double* partial = nullptr;
+    SorCtl* h_ctl = nullptr;  // pinned, kSlots entries
+    Diag* h_diag = nullptr;   // pinned
+    hipEvent_t ev[kSlots] = {};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    // defect correction
+    float* e32[2] = {nullptr, nullptr};
+    float* r32 = nullptr;
+    // launch shape
+    TileShape shape{32, 64, 4};
+    int tiles_i = 0, tiles_j = 0;
+    int launches_per_poll = 32;
+    int k_prev = 0;       // sweeps of the previous solve (queue-ahead prediction)
+    WallBC bc{};
+    char err[256] = "";
+};
+
+static int fail(cnv_ctx* c, int code, const char* fmt, ...) {
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) snprintf(c->err, sizeof(c->err), "%s", buf);
+    snprintf(g_err, sizeof(g_err), "%s", buf);
+    return code;
+}
+
+#define HIPC(c, x)                                                                              \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) return fail((c), CNV_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+#define LAUNCH(c, x)                                                                           \
+    do {                                                                                        \
+        if ((x) != 0) return fail((c), CNV_ERR_HIP, "launch failed: %s (%s)", #x,             \
+                                  hipGetErrorString(hipGetLastError()));                        \
+    } while (0)
+
+// The public functions below are declared extern "C" in cnv.h, which gives them C linkage.
+
+double cnv_reference_beta(int nx, int ny) {
+    const double pi = 3.14159265359;  // include/poisson.h:9 of the reference
+    return 0.5 * (2 / (1 + sin(pi / (nx + 1))) + 2 / (1 + sin(pi / (ny + 1))));  // src/main.c:134
+}
+
+void cnv_default_params(cnv_params* p, int n) {
+    memset(p, 0, sizeof(*p));
+    p->n = n;
+    p->order = 6;
+    p->poisson_type = 2;
+    p->precision = CNV_PREC_FP64;
+    p->itmax = 10000;
+    p->device = 0;
+    p->dx = 1.0 / n;
+    p->dy = 1.0 / n;
+    p->Re = 1000.0;
+    p->dt = 0.005;
+    p->beta = cnv_reference_beta(n, n);
+    p->tol = 1e-3;
+    p->u4 = 1.0;
+    p->inner_rtol = 1e-5;
+    p->max_outer = 50;
+    p->diagnostics = 1;
+}
+
+const char* cnv_status_string(int s) {
+    switch (s) {
+        case CNV_OK: return "ok";
+        case CNV_ERR_ARG: return "invalid argument";
+        case CNV_ERR_HIP: return "HIP runtime error";
+        case CNV_NOT_CONVERGED: return "Poisson solve did not converge within poisson_max_it";
+        case CNV_ERR_NOMEM: return "out of memory";
+        case CNV_ERR_NO_DEVICE: return "no HIP device";
+        default: return "unknown status";
+    }
+}
+
+const char* cnv_last_error(const cnv_ctx* c) { return c ? c->err : g_err; }
+
+int cnv_device_name(int device, char* buf, int len) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(nullptr, CNV_ERR_NO_DEVICE, "no device %d", device);
+    snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+    return CNV_OK;
+}
+
+static TileShape shape_for(int spl) {
+    switch (spl) {
+        case 1: return TileShape{32, 64, 1};
+        case 2: return TileShape{32, 64, 2};
+        case 8: return TileShape{32, 64, 8};
+        default: return TileShape{32, 64, 4};
+    }
+}
+
+static int upload_tables(cnv_ctx* c) {
+    const int n = c->n;
+    const int o = c->p.order;
+    std::vector<double> D((size_t)n * n);
+    std::vector<int> start(n), start2(n);
+    std::vector<double> tab((size_t)4 * n * kStencilWidth);
+    const double hs[4] = {c->p.dx, c->p.dy, c->p.dx, c->p.dy};
+    const int which[4] = {1, 1, 2, 2};
+    for (int t = 0; t < 4; ++t) {
+        int rc = cnv_fd_dense(which[t], n, o, hs[t], D.data());
+        if (rc) return fail(c, CNV_ERR_ARG, "** Error: valid orders are 2, 4 or 6 ** (order=%d, n=%d)", o, n);
+        if (cnv_fd_windows(n, D.data(), t == 0 ? start.data() : start2.data(), tab.data() + (size_t)t * n * kStencilWidth))
+            return fail(c, CNV_ERR_ARG, "stencil window does not cover a row (n=%d)", n);
+    }
+    std::vector<float> ftab(tab.size());
+    for (size_t q = 0; q < tab.size(); ++q) ftab[q] = (float)tab[q];
+    HIPC(c, hipMalloc(&c->d_start, sizeof(int) * n));
+    HIPC(c, hipMalloc(&c->d_tab, sizeof(double) * tab.size()));
+    HIPC(c, hipMalloc(&c->d_ftab, sizeof(float) * ftab.size()));
+    HIPC(c, hipMemcpy(c->d_start, start.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->d_ftab, ftab.data(), sizeof(float) * ftab.size(), hipMemcpyHostToDevice));
+    const size_t stride = (size_t)n * kStencilWidth;
+    c->st.start = c->d_start;
+    c->st.d1x = c->d_tab;
+    c->st.d1y = c->d_tab + stride;
+    c->st.d2x = c->d_tab + 2 * stride;
+    c->st.d2y = c->d_tab + 3 * stride;
+    c->st.f1x = c->d_ftab;
+    c->st.f1y = c->d_ftab + stride;
+    c->st.f2x = c->d_ftab + 2 * stride;
+    c->st.f2y = c->d_ftab + 3 * stride;
+    return CNV_OK;
+}
+
+int cnv_set_state(cnv_ctx* c, const double* w, const double* psi, const double* u, const double* v) {
+    const size_t bytes = sizeof(double) * c->cells;
+    if (w) HIPC(c, hipMemcpyAsync(c->w[c->wcur], w, bytes, hipMemcpyHostToDevice, c->stream));
+    if (psi) {
+        // psi lives in psi[parity]; make both buffers hold it so that any parity reads it
+        HIPC(c, hipMemcpyAsync(c->psi[0], psi, bytes, hipMemcpyHostToDevice, c->stream));
+        HIPC(c, hipMemcpyAsync(c->psi[1], psi, bytes, hipMemcpyHostToDevice, c->stream));
+    }
+    if (u) HIPC(c, hipMemcpyAsync(c->u, u, bytes, hipMemcpyHostToDevice, c->stream));
+    if (v) HIPC(c, hipMemcpyAsync(c->v, v, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return CNV_OK;
+}
+
+static int current_psi(cnv_ctx* c, const double** out) {
+    if (c->p.precision == CNV_PREC_FP32_DC) {
+        *out = c->psi[0];
+        return CNV_OK;
+    }
+    HIPC(c, hipMemcpyAsync(c->h_ctl, c->ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    *out = c->psi[c->h_ctl[0].parity & 1];
+    return CNV_OK;
+}
+
+int cnv_get_state(cnv_ctx* c, double* w, double* psi, double* u, double* v) {
+    const size_t bytes = sizeof(double) * c->cells;
+    if (psi) {
+        const double* src = nullptr;
+        int rc = current_psi(c, &src);
+        if (rc) return rc;
+        HIPC(c, hipMemcpyAsync(psi, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (w) HIPC(c, hipMemcpyAsync(w, c->w[c->wcur], bytes, hipMemcpyDeviceToHost, c->stream));
+    if (u) HIPC(c, hipMemcpyAsync(u, c->u, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (v) HIPC(c, hipMemcpyAsync(v, c->v, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return CNV_OK;
+}
+
+int cnv_device_state(cnv_ctx* c, double** w, double** psi, double** u, double** v) {
+    if (psi) {
+        const double* src = nullptr;
+        int rc = current_psi(c, &src);
+        if (rc) return rc;
+        *psi = const_cast<double*>(src);
+    }
+    if (w) *w = c->w[c->wcur];
+    if (u) *u = c->u;
+    if (v) *v = c->v;
+    return CNV_OK;
+}
+
+static int initial_state(cnv_ctx* c) {
+    // w = psi = 0; u = ui, v = vi on the interior and 0 on the edges (src/main.c:178-221)
+    std::vector<double> a(c->cells, 0.0), uu(c->cells, 0.0), vv(c->cells, 0.0);
+    const int n = c->n;
+    for (int i = 1; i < n - 1; ++i)
+        for (int j = 1; j < n - 1; ++j) {
+            uu[(size_t)i * n + j] = c->p.ui;
+            vv[(size_t)i * n + j] = c->p.vi;
+        }
+    return cnv_set_state(c, a.data(), a.data(), uu.data(), vv.data());
+}
+
+void cnv_destroy(cnv_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* dev[] = {c->w[0], c->w[1], c->psi[0], c->psi[1], c->u, c->v, c->d_start, c->d_tab,
+                   c->d_ftab, c->ctl, c->diag, c->partial, c->e32[0], c->e32[1], c->r32};
+    for (void* p : dev)
+        if (p) (void)hipFree(p);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->h_diag) (void)hipHostFree(c->h_diag);
+    for (int s = 0; s < kSlots; ++s)
+        if (c->ev[s]) (void)hipEventDestroy(c->ev[s]);
+    if (c->t0) (void)hipEventDestroy(c->t0);
+    if (c->t1) (void)hipEventDestroy(c->t1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int cnv_create(const cnv_params* p, cnv_ctx** out) {
+    *out = nullptr;
+    if (!p) return fail(nullptr, CNV_ERR_ARG, "null params");
+    if (p->n < 8) return fail(nullptr, CNV_ERR_ARG, "grid side n=%d: need n >= 8", p->n);
+    if (p->order != 2 && p->order != 4 && p->order != 6)
+        return fail(nullptr, CNV_ERR_ARG, "** Error: valid orders are 2, 4 or 6 ** (order=%d)", p->order);
+    if (p->poisson_type != 1 && p->poisson_type != 2)
+        return fail(nullptr, CNV_ERR_ARG, "Error - invalid option for Poisson solver (poisson_type=%d)", p->poisson_type);
+    if (p->precision != CNV_PREC_FP64 && p->precision != CNV_PREC_FP32_DC)
+        return fail(nullptr, CNV_ERR_ARG, "invalid precision %d", p->precision);
+    if (!(p->dx > 0) || !(p->dy > 0)) return fail(nullptr, CNV_ERR_ARG, "grid spacing must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p->device || p->device < 0)
+        return fail(nullptr, CNV_ERR_NO_DEVICE, "HIP device %d not available (%d devices)", p->device, ndev);
+    if (hipSetDevice(p->device) != hipSuccess) return fail(nullptr, CNV_ERR_NO_DEVICE, "hipSetDevice(%d) failed", p->device);
+
+    cnv_ctx* c = new cnv_ctx();
+    c->p = *p;
+    c->n = p->n;
+    c->ld = p->n;
+    c->cells = (size_t)p->n * p->n;
+    c->bc = WallBC{p->u1, p->u2, p->u3, p->u4, p->v1, p->v2, p->v3, p->v4};
+    int spl = p->sweeps_per_launch ? p->sweeps_per_launch : 4;
+    c->shape = shape_for(spl);
+    if (p->sweeps_per_launch && c->shape.s != p->sweeps_per_launch) {
+        cnv_destroy(c);
+        return fail(nullptr, CNV_ERR_ARG, "sweeps_per_launch must be 1, 2, 4 or 8");
+    }
+    c->launches_per_poll = p->launches_per_poll > 0 ? p->launches_per_poll : 32;
+    c->tiles_i = (c->n + c->shape.ti - 1) / c->shape.ti;
+    c->tiles_j = (c->n + c->shape.tj - 1) / c->shape.tj;
+    auto setup = [&]() -> int {
+        HIPC(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        const size_t bytes = sizeof(double) * c->cells;
+        for (int b = 0; b < 2; ++b) {
+            HIPC(c, hipMalloc(&c->w[b], bytes));
+            HIPC(c, hipMalloc(&c->psi[b], bytes));
+        }
+        HIPC(c, hipMalloc(&c->u, bytes));
+        HIPC(c, hipMalloc(&c->v, bytes));
+        HIPC(c, hipMalloc(&c->ctl, sizeof(SorCtl)));
+        HIPC(c, hipMalloc(&c->diag, sizeof(Diag)));
+        HIPC(c, hipMemset(c->ctl, 0, sizeof(SorCtl)));
+        HIPC(c, hipMemset(c->diag, 0, sizeof(Diag)));
+        HIPC(c, hipMalloc(&c->partial, sizeof(double) * 8 * (size_t)c->tiles_i * c->tiles_j));
+        HIPC(c, hipHostMalloc(&c->h_ctl, sizeof(SorCtl) * kSlots, hipHostMallocDefault));
+        HIPC(c, hipHostMalloc(&c->h_diag, sizeof(Diag), hipHostMallocDefault));
+        for (int s = 0; s < kSlots; ++s) HIPC(c, hipEventCreateWithFlags(&c->ev[s], hipEventDisableTiming));
+        HIPC(c, hipEventCreate(&c->t0));
+        HIPC(c, hipEventCreate(&c->t1));
+        if (p->precision == CNV_PREC_FP32_DC) {
+            for (int b = 0; b < 2; ++b) HIPC(c, hipMalloc(&c->e32[b], sizeof(float) * c->cells));
+            HIPC(c, hipMalloc(&c->r32, sizeof(float) * c->cells));
+        }
+        int rc = upload_tables(c);
+        if (rc) return rc;
+        return initial_state(c);
+    };
+    int rc = setup();
+    if (rc) {
+        char keep[256];
+        snprintf(keep, sizeof(keep), "%s", c->err);
+        cnv_destroy(c);
+        snprintf(g_err, sizeof(g_err), "%s", keep);
+        return rc;
+    }
+    *out = c;
+    return CNV_OK;
+}
+
+int cnv_sweeps_per_launch(const cnv_ctx* c) { return c->shape.s; }
+
+// ---- SOR driver ---------------------------------------------------------------------------------
+template <typename T>
+static SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int negate) {
+    SorArgs<T> a{};
+    a.src_f = src;
+    a.psi0 = b0;
+    a.psi1 = b1;
+    a.partial = c->partial;
+    a.ctl = c->ctl;
+    a.nr = c->n;
+    a.nc = c->n;
+    a.ld = c->ld;
+    a.tiles_i = c->tiles_i;
+    a.tiles_j = c->tiles_j;
+    a.negate = negate;
+    const double dx = c->p.dx, dy = c->p.dy;
+    // the reference evaluates dy*dy, dx*dx, dx*dx*dy*dy and 2*(dx*dx+dy*dy) left to right
+    a.c1 = (T)(dy * dy);
+    a.c2 = (T)(dx * dx);
+    a.c3 = (T)(dx * dx * dy * dy);
+    a.B = (T)(2 * (dx * dx + dy * dy));
+    a.beta = (T)c->p.beta;
+    a.omb = (T)(1 - c->p.beta);
+    return a;
+}
+
+// Queue SOR launches until the device control block reports convergence or failure.  Keeps
+// one chunk queued behind the one being waited on, so the GPU never idles on the host.
+static int launch_sor(cnv_ctx* c, const SorArgs<double>& a, bool gs) { return sor_fp64_launch(a, c->shape, gs, c->stream); }
+static int launch_sor(cnv_ctx* c, const SorArgs<float>& a, bool gs) { return sor_fp32_launch(a, c->shape, gs, c->stream); }
+
+template <typename T>
+static int run_sor(cnv_ctx* c, const SorArgs<T>& a, SorCtl* final_ctl) {
+    const bool gs = (c->p.poisson_type == 1);
+    const int S = c->shape.s;
+    const int M = c->launches_per_poll;
+    const long long cap = (long long)c->p.itmax + 4LL * M + 8;  // working launches <= itmax/S + 1
+    int first = M;
+    if (c->k_prev > 0) {
+        const int predicted = c->k_prev / S - M;
+        if (predicted > first) first = predicted;
+    }
+    long long queued = 0;
+    int slot = 0;
+    int pending[kSlots];
+    int npend = 0;
+    while (true) {
+        const int chunk = queued == 0 ? first : M;
+        for (int l = 0; l < chunk; ++l) {
+            if (launch_sor(c, a, gs)) return fail(c, CNV_ERR_HIP, "SOR launch failed: %s", hipGetErrorString(hipGetLastError()));
+        }
+        queued += chunk;
+        HIPC(c, hipMemcpyAsync(&c->h_ctl[slot], c->ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipEventRecord(c->ev[slot], c->stream));
+        pending[npend++] = slot;
+        slot = (slot + 1) % kSlots;
+        if (npend >= 2) {
+            const int s0 = pending[0];
+            for (int q = 1; q < npend; ++q) pending[q - 1] = pending[q];
+            --npend;
+            HIPC(c, hipEventSynchronize(c->ev[s0]));
+            const SorCtl snap = c->h_ctl[s0];
+            if (snap.status == SOR_CONVERGED || snap.status == SOR_FAILED) {
+                *final_ctl = snap;
+                return CNV_OK;
+            }
+        }
+        if (queued > cap + first) return fail(c, CNV_ERR_HIP, "SOR control block did not terminate after %lld launches", queued);
+    }
+}
+
+static int poisson_fp64(cnv_ctx* c, const double* f_src, int negate, SorCtl* fin) {
+    LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, c->p.tol, c->p.itmax, c->stream));
+    SorArgs<double> a = sor_args<double>(c, f_src, c->psi[0], c->psi[1], negate);
+    int rc = run_sor<double>(c, a, fin);
+    if (rc) return rc;
+    c->k_prev = fin->k;
+    return CNV_OK;
+}
+
+// defect correction: psi[0] accumulates in FP64, e32 ping-pong holds the inner FP32 iterates
+static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer, int* inner_sweeps) {
+    LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, c->p.tol, c->p.itmax, c->stream));
+    const double dx = c->p.dx, dy = c->p.dy;
+    const double c1 = dy * dy, c2 = dx * dx, c3 = dx * dx * dy * dy, B = 2 * (dx * dx + dy * dy);
+    SorArgs<float> a = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
+    int total = 0;
+    const int max_outer = c->p.max_outer > 0 ? c->p.max_outer : 50;
+    for (int o = 0; o < max_outer; ++o) {
+        LAUNCH(c, dc_residual_launch(w_src, c->psi[0], c->r32, c->diag, c->n, c->ld, c1, c2, c3, B, c->p.beta, c->stream));
+        const int budget = c->p.itmax - total;
+        LAUNCH(c, dc_inner_init_launch(c->e32[0], c->ctl, c->n, c->ld, c->p.inner_rtol, c->p.tol, budget, c->stream));
+        // e32[1] is written in full by the first launch; its edges must read as zero
+        SorCtl snap{};
+        int rc = run_sor<float>(c, a, &snap);
+        if (rc) return rc;
+        LAUNCH(c, dc_correct_launch(c->psi[0], c->e32[0], c->e32[1], c->ctl, c->n, c->ld, c->stream));
+        total += snap.k;
+        *outer = o + 1;
+        *fin = snap;
+        fin->k = total;
+        if (snap.status == SOR_FAILED) return CNV_OK;
+        if (snap.e_last < c->p.tol) {
+            *inner_sweeps = total;
+            return CNV_OK;
+        }
+    }
+    *inner_sweeps = total;
+    fin->status = SOR_FAILED;
+    return CNV_OK;
+}
+
+static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
+    const int n = c->n, ld = c->ld;
+    const double rRe = 1. / c->p.Re;  // src/fluiddyn.c:98 "(1. / Re)"
+    double* w_in = c->w[c->wcur];
+    double* w_out = c->w[c->wcur ^ 1];
+    const bool dc = c->p.precision == CNV_PREC_FP32_DC;
+    LAUNCH(c, wall_vorticity_launch(w_in, c->u, c->v, n, ld, c->st, c->bc, c->stream));
+    if (dc)
+        LAUNCH(c, transport_fp32_launch(w_in, w_out, c->u, c->v, n, ld, c->st, c->bc, rRe, c->p.dt, c->stream));
+    else
+        LAUNCH(c, transport_fp64_launch(w_in, w_out, c->u, c->v, n, ld, c->st, c->bc, rRe, c->p.dt, c->stream));
+    c->wcur ^= 1;
+    SorCtl fin{};
+    int outer = 0, inner = 0;
+    int rc = dc ? poisson_dc(c, w_out, &fin, &outer, &inner) : poisson_fp64(c, w_out, 1, &fin);
+    if (rc) return rc;
+    if (dc)
+        LAUNCH(c, velocity_fp32_launch(c->psi[0], c->u, c->v, n, ld, c->st, c->stream));
+    else
+        LAUNCH(c, velocity_launch(c->psi[0], c->psi[1], c->ctl, c->u, c->v, n, ld, c->st, c->stream));
+    if (c->p.diagnostics) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, n, ld, c->st, c->stream));
+    cnv_step_stats s{};
+    s.status = fin.status == SOR_CONVERGED ? CNV_OK : CNV_NOT_CONVERGED;
+    s.sweeps = fin.k;
+    s.k = fin.k - 1;
+    s.e = fin.e_last;
+    s.outer = outer;
+    if (c->p.diagnostics) {
+        HIPC(c, hipMemcpyAsync(c->h_diag, c->diag, sizeof(Diag), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        s.cont_max = dkey_inv(c->h_diag->kmax);
+        s.cont_min = dkey_inv(c->h_diag->kmin);
+    }
+    if (stats) *stats = s;
+    if (s.status != CNV_OK)
+        return fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
+    return CNV_OK;
+}
+
+int cnv_step(cnv_ctx* c, int nsteps, cnv_step_stats* stats) {
+    if (!c || nsteps < 0) return fail(c, CNV_ERR_ARG, "bad arguments to cnv_step");
+    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    for (int t = 0; t < nsteps; ++t) {
+        int rc = one_step(c, stats ? &stats[t] : nullptr);
+        if (rc) return rc;
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return CNV_OK;
+}
+
+int cnv_step_timed(cnv_ctx* c, int nsteps, cnv_step_stats* stats, double* elapsed_ms) {
+    if (!c || nsteps < 0) return fail(c, CNV_ERR_ARG, "bad arguments to cnv_step_timed");
+    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    HIPC(c, hipStreamSynchronize(c->stream));
+    HIPC(c, hipEventRecord(c->t0, c->stream));
+    for (int t = 0; t < nsteps; ++t) {
+        int rc = one_step(c, stats ? &stats[t] : nullptr);
+        if (rc) return rc;
+    }
+    HIPC(c, hipEventRecord(c->t1, c->stream));
+    HIPC(c, hipEventSynchronize(c->t1));
+    float ms = 0.f;
+    HIPC(c, hipEventElapsedTime(&ms, c->t0, c->t1));
+    if (elapsed_ms) *elapsed_ms = ms;
+    return CNV_OK;
+}
+
+int cnv_poisson_device(cnv_ctx* c, const double* d_f, double* d_psi, int* k, double* e) {
+    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    SorCtl fin{};
+    int rc = poisson_fp64(c, d_f, 0, &fin);
+    if (rc) return rc;
+    HIPC(c, hipMemcpyAsync(d_psi, c->psi[fin.parity & 1], sizeof(double) * c->cells, hipMemcpyDeviceToDevice, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (k) *k = fin.k - 1;
+    if (e) *e = fin.e_last;
+    if (fin.status != SOR_CONVERGED)
+        return fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
+    return CNV_OK;
+}
+
+int cnv_poisson_host(cnv_ctx* c, const double* f, double* psi, int* k, double* e) {
+    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    const size_t bytes = sizeof(double) * c->cells;
+    // stage f in the spare w buffer: the solve reads it as its source term directly
+    double* d_f = c->w[c->wcur ^ 1];
+    HIPC(c, hipMemcpyAsync(d_f, f, bytes, hipMemcpyHostToDevice, c->stream));
+    SorCtl fin{};
+    int rc = poisson_fp64(c, d_f, 0, &fin);
+    if (rc) return rc;
+    HIPC(c, hipMemcpyAsync(psi, c->psi[fin.parity & 1], bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (k) *k = fin.k - 1;
+    if (e) *e = fin.e_last;
+    if (fin.status != SOR_CONVERGED)
+        return fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
+    return CNV_OK;
+}
+
+int cnv_time_sor(cnv_ctx* c, int launches, double* mean_ms) {
+    if (!c || launches <= 0) return fail(c, CNV_ERR_ARG, "bad arguments to cnv_time_sor");
+    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    const bool dc = c->p.precision == CNV_PREC_FP32_DC;
+    // a solve that never converges (tol < 0) and never runs out of budget, on the current w
+    if (dc) {
+        LAUNCH(c, dc_residual_launch(c->w[c->wcur], c->psi[0], c->r32, c->diag, c->n, c->ld, c->p.dy * c->p.dy,
+                                     c->p.dx * c->p.dx, c->p.dx * c->p.dx * c->p.dy * c->p.dy,
+                                     2 * (c->p.dx * c->p.dx + c->p.dy * c->p.dy), c->p.beta, c->stream));
+        LAUNCH(c, dc_inner_init_launch(c->e32[0], c->ctl, c->n, c->ld, 0.0, -1.0, 0x7fffffff, c->stream));
+    } else {
+        LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, -1.0, 0x7fffffff, c->stream));
+    }
+    SorArgs<double> a64 = sor_args<double>(c, c->w[c->wcur], c->psi[0], c->psi[1], 1);
+    SorArgs<float> a32 = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
+    const bool gs = c->p.poisson_type == 1;
+    // warm the instruction cache and clocks, then time
+    for (int l = 0; l < 8; ++l)
+        LAUNCH(c, dc ? sor_fp32_launch(a32, c->shape, gs, c->stream) : sor_fp64_launch(a64, c->shape, gs, c->stream));
+    HIPC(c, hipEventRecord(c->t0, c->stream));
+    for (int l = 0; l < launches; ++l)
+        LAUNCH(c, dc ? sor_fp32_launch(a32, c->shape, gs, c->stream) : sor_fp64_launch(a64, c->shape, gs, c->stream));
+    HIPC(c, hipEventRecord(c->t1, c->stream));
+    HIPC(c, hipEventSynchronize(c->t1));
+    float ms = 0.f;
+    HIPC(c, hipEventElapsedTime(&ms, c->t0, c->t1));
+    *mean_ms = ms / launches;
+    return CNV_OK;
+}
+
+int cnv_euler_host(int n, double* w, const double* dwdx, const double* dwdy, const double* d2wdx2,
+                   const double* d2wdy2, const double* u, const double* v, double Re, double dt) {
+    if (n <= 0) return fail(nullptr, CNV_ERR_ARG, "bad size");
+    const size_t bytes = sizeof(double) * (size_t)n;
+    double* d[7] = {};
+    const double* h[7] = {w, dwdx, dwdy, d2wdx2, d2wdy2, u, v};
+    int rc = CNV_OK;
+    for (int q = 0; q < 7 && rc == CNV_OK; ++q) {
+        if (hipMalloc(&d[q], bytes) != hipSuccess || hipMemcpy(d[q], h[q], bytes, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(nullptr, CNV_ERR_HIP, "euler: device allocation/upload failed");
+    }
+    if (rc == CNV_OK) {
+        if (euler_launch(d[0], d[1], d[2], d[3], d[4], d[5], d[6], 1. / Re, dt, n, nullptr) ||
+            hipMemcpy(w, d[0], bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(nullptr, CNV_ERR_HIP, "euler: kernel failed");
+    }
+    for (int q = 0; q < 7; ++q)
+        if (d[q]) (void)hipFree(d[q]);
+    return rc;
+}
+
+static int elementwise_host(int m, int n, const double* a, const double* b, double* out, int sub) {
+    const size_t cnt = (size_t)m * n;
+    const size_t bytes = sizeof(double) * cnt;
+    double *da = nullptr, *db = nullptr, *dc = nullptr;
+    int rc = CNV_OK;
+    if (hipMalloc(&da, bytes) != hipSuccess || hipMalloc(&db, bytes) != hipSuccess || hipMalloc(&dc, bytes) != hipSuccess ||
+        hipMemcpy(da, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess || add_launch(da, db, dc, (long long)cnt, sub, nullptr) ||
+        hipMemcpy(out, dc, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(nullptr, CNV_ERR_HIP, "elementwise kernel failed");
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dc) (void)hipFree(dc);
+    return rc;
+}
+
+int cnv_add_host(int m, int n, const double* a, const double* b, double* out) {
+    if (m <= 0 || n <= 0) return fail(nullptr, CNV_ERR_ARG, "bad size");
+    return elementwise_host(m, n, a, b, out, 0);
+}
+int cnv_sub_host(int m, int n, const double* a, const double* b, double* out) {
+    if (m <= 0 || n <= 0) return fail(nullptr, CNV_ERR_ARG, "bad size");
+    return elementwise_host(m, n, a, b, out, 1);
+}

@@ -1,0 +1,119 @@
+/*
+ * cnv.h -- device-resident C ABI of the MI355X (gfx950) vorticity-streamfunction step.
+ *
+ * This is the layer the cnavier driver uses at scale.  It has no counterpart with these names in
+ * the reference: the reference's step is inline in src/main.c:276-451 and calls euler()
+ * (include/fluiddyn.h:8), poisson_SOR_log()/poisson_log() (include/poisson.h:15-16) and the
+ * dense mtrxmul() (include/linearalg.h:30) on host matrices.  The legacy headers in this
+ * directory (poisson.h, finitediff.h, fluiddyn.h, linearalg.h, config.h, utils.h) keep those
+ * exact signatures and are implemented on top of this API.
+ *
+ * Conventions: one context = one cavity on one GPU, one HIP stream (default priority), device
+ * buffers owned by the context.  Grids are n x n binary64, row-major, element (i, j) at
+ * i*n + j, row i is y and column j is x (src/main.c:283-296).  Every call returns an int status
+ * (CNV_OK = 0); cnv_last_error() gives the message.  A context is not thread-safe.
+ */
+#ifndef CNV_H
+#define CNV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CNV_OK 0
+#define CNV_ERR_ARG 1          /* bad argument (size, order, type, precision ...)           */
+#define CNV_ERR_HIP 2          /* a HIP runtime call failed                                 */
+#define CNV_NOT_CONVERGED 3    /* Poisson hit itmax (src/poisson.c:280-284 would exit(1))   */
+#define CNV_ERR_NOMEM 4
+#define CNV_ERR_NO_DEVICE 5
+
+#define CNV_PREC_FP64 0        /* plain FP64, bit-faithful red-black SOR                      */
+#define CNV_PREC_FP32_DC 1     /* FP32 stencils and SOR sweeps inside FP64 defect correction */
+
+typedef struct cnv_ctx cnv_ctx;
+
+typedef struct {
+    int n;                 /* nx == ny (src/main.c:142-152 needs square), n >= 8            */
+    int order;             /* finite-difference order 2, 4 or 6 (src/finitediff.c)          */
+    int poisson_type;      /* 1 = Gauss-Seidel, 2 = SOR (src/main.c:349-361)                */
+    int precision;         /* CNV_PREC_FP64 or CNV_PREC_FP32_DC                             */
+    int itmax;             /* poisson_max_it: sweeps allowed per solve                       */
+    int device;            /* HIP device ordinal                                            */
+    double dx, dy;         /* Lx/nx and Ly/ny (src/main.c:138-139)                          */
+    double Re, dt;
+    double beta;           /* SOR relaxation, src/main.c:134 by default                      */
+    double tol;            /* stop when the L1 norm of a sweep's update < tol (src/poisson.c:272-273) */
+    double ui, vi;         /* initial interior velocity (src/main.c:214-221)                */
+    double u1, u2, u3, u4; /* wall velocities: column 0, column n-1, row 0, row n-1          */
+    double v1, v2, v3, v4;
+    /* FP32_DC only */
+    double inner_rtol;     /* inner FP32 solve: stop at this fraction of its first update    */
+    int max_outer;         /* defect-correction iterations allowed per solve                */
+    /* tuning (0 = automatic) */
+    int sweeps_per_launch; /* SOR sweeps fused per kernel launch (1, 2, 4 or 8)              */
+    int launches_per_poll; /* SOR launches queued between host polls of the convergence flag */
+    int diagnostics;       /* 1 = continuity max/min every step (src/main.c:387-408)        */
+} cnv_params;
+
+typedef struct {
+    int status;            /* CNV_OK or CNV_NOT_CONVERGED                                    */
+    int k;                 /* logged Poisson index = sweeps - 1 (src/poisson.c:275)          */
+    int sweeps;            /* SOR sweeps of this step (FP64) / inner FP32 sweeps (DC)        */
+    int outer;             /* defect-correction iterations (DC), 0 for FP64                  */
+    double e;              /* L1 update of the last sweep (FP64) / outer norm (DC)           */
+    double cont_max, cont_min; /* continuity diagnostic, if enabled                          */
+} cnv_step_stats;
+
+/* Reference defaults for an n x n cavity: load_default_config() values (src/config.c:47-87)
+ * except n, with beta from src/main.c:134 and dx = 1/n. */
+void cnv_default_params(cnv_params *p, int n);
+double cnv_reference_beta(int nx, int ny);
+
+int cnv_create(const cnv_params *p, cnv_ctx **out);
+void cnv_destroy(cnv_ctx *ctx);
+const char *cnv_last_error(const cnv_ctx *ctx);   /* ctx may be NULL: last global error */
+const char *cnv_status_string(int status);
+
+/* State transfer (host arrays of n*n doubles; any pointer may be NULL to skip a field).
+ * A new context holds the reference initial condition (src/main.c:178-221). */
+int cnv_set_state(cnv_ctx *ctx, const double *w, const double *psi, const double *u, const double *v);
+int cnv_get_state(cnv_ctx *ctx, double *w, double *psi, double *u, double *v);
+
+/* Advance nsteps time steps (src/main.c:276-451).  stats: nsteps entries or NULL.  Stops at the
+ * first step whose Poisson solve does not converge and returns CNV_NOT_CONVERGED. */
+int cnv_step(cnv_ctx *ctx, int nsteps, cnv_step_stats *stats);
+
+/* Timed variant: wall time of the step loop measured with HIP events on the context stream. */
+int cnv_step_timed(cnv_ctx *ctx, int nsteps, cnv_step_stats *stats, double *elapsed_ms);
+
+/* Device pointers of the state (w, psi, u, v) for zero-copy callers; valid until the next step. */
+int cnv_device_state(cnv_ctx *ctx, double **w, double **psi, double **u, double **v);
+
+/* Poisson solve alone on this context's grid: psi = SOR(f) from zero (src/poisson.c:224-285),
+ * f given as a host array; psi written to host.  k/e as logged by the reference. */
+int cnv_poisson_host(cnv_ctx *ctx, const double *f, double *psi, int *k, double *e);
+
+/* Same on device pointers (f, psi: n*n doubles in device memory), on the context stream. */
+int cnv_poisson_device(cnv_ctx *ctx, const double *d_f, double *d_psi, int *k, double *e);
+
+/* Explicit Euler update alone (src/fluiddyn.c:71-102) on host arrays of n*n doubles. */
+int cnv_euler_host(int n, double *w, const double *dwdx, const double *dwdy, const double *d2wdx2,
+                   const double *d2wdy2, const double *u, const double *v, double Re, double dt);
+
+/* Continuity field Dx u + Dy v's pointwise sum (src/fluiddyn.c:126-154) and vorticity's
+ * second-minus-first (src/fluiddyn.c:179-207) on host arrays: out = a + b / out = b - a. */
+int cnv_add_host(int m, int n, const double *a, const double *b, double *out);
+int cnv_sub_host(int m, int n, const double *a, const double *b, double *out);
+
+/* Measurement hooks used by bench.py.
+ * cnv_time_sor: replays `launches` SOR kernel launches (sweeps_per_launch sweeps each) on the
+ * current f with convergence disabled and returns the mean launch duration in ms, measured with
+ * HIP events on the stream the kernel runs on.  Restores nothing: call on a scratch context. */
+int cnv_time_sor(cnv_ctx *ctx, int launches, double *mean_ms);
+int cnv_sweeps_per_launch(const cnv_ctx *ctx);
+int cnv_device_name(int device, char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
