@@ -1,0 +1,208 @@
+"""Benchmark: Mcell-steps/s and %HBM roofline of the 1024^2 Re=1000 lid-driven cavity step.
+
+Workload (the metric's configuration): 1024 x 1024 cavity on the unit square, Re = 1000, lid
+velocity 1, dt = 0.2*min(h^2 Re/4, h) = 50*2^-20, 6th-order stencils (the reference default),
+plain FP64 red-black SOR with omega = 1.95 to an L1 update below 1e-10, psi = omega = 0 at t = 0.
+A "step" is one full time step (wall vorticity, transport + Euler, Poisson solve, velocities,
+continuity diagnostic).  The same steps are then run in the FP32 + FP64 defect-correction form
+and compared with the FP64 run.
+
+Prints one JSON line.  value = cells * steps / wall time of the timed steps (HIP events on the
+solver stream), inputs resident in HBM.  roofline: the SOR kernel (the dominant one), algorithmic
+bytes per launch = sweeps_per_launch * 24 B * cells (read psi, read omega, write psi per sweep)
+over its mean launch duration, measured with HIP events on the stream it runs on.
+cpu_baseline: the oracle (the repository's CPU restatement of the reference step), OpenMP
+red-black, on a bounded sample of the same workload (see "sample").
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fluid-dynamics1_amd"))
+sys.path.insert(0, ROOT)
+
+N = 1024
+RE = 1000.0
+DT = 50 * 2.0 ** -20          # 0.2 * min(h^2 Re / 4, h) with h = 1/1024
+OMEGA = 1.95
+TOL = 1e-10
+HBM_PEAK = 8.0e12             # MI355X HBM3E, bytes/s
+BYTES_PER_CELL_SWEEP = 24     # read psi, read omega, write psi (FP64)
+BYTES_PER_CELL_SWEEP_FP32 = 12
+
+
+def workload_params(cnv, precision, args):
+    return cnv.default_params(N, Re=RE, dt=DT, order=6, beta=OMEGA, tol=TOL, itmax=50_000_000,
+                              precision=precision, sweeps_per_launch=args.spl, tile_waves=args.tile_waves,
+                              wave_rows=args.wave_rows, diagnostics=1)
+
+
+def timed_steps(cav, k, tag):
+    """k steps, each bracketed by HIP events on the solver stream; returns (stats, total ms)."""
+    stats, total = [], 0.0
+    for t in range(k):
+        st, ms = cav.step_timed(1)
+        stats += st
+        total += ms
+        print("bench: %s step %d: %.2f ms, %d sweeps" % (tag, t, ms, st[0].sweeps), file=sys.stderr, flush=True)
+    return stats, total
+
+
+def read_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if one exists."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(sample_seconds=12.0):
+    """Oracle at the same workload on the host cores: the first time step's transport plus a
+    bounded number of red-black SOR sweeps, timed; scaled to cell-steps/s with the GPU run's
+    mean sweeps per step (identical arithmetic gives identical sweep counts)."""
+    from oracle import orc
+    threads = min(16, os.cpu_count() or 1)
+    p = orc.make_params(N, Re=RE, dt=DT, order=6, tol=-1.0, itmax=1, beta_=OMEGA,
+                        ordering=orc.REDBLACK, threads=threads)
+    s = orc.State(p)
+    orc.lib().orc_apply_velocity_bc(orc.C.byref(p), s.u, s.v)
+    orc.lib().orc_wall_vorticity(orc.C.byref(p), s.u, s.v, s.w)
+    t0 = time.perf_counter()
+    orc.lib().orc_transport(orc.C.byref(p), s.w, s.u, s.v)
+    t_transport = time.perf_counter() - t0
+    f = -s.w
+    # calibrate, then run a sample of about sample_seconds
+    p.itmax = 4
+    t0 = time.perf_counter()
+    orc.poisson(p, f)
+    per = (time.perf_counter() - t0) / 4
+    sweeps = int(max(8, min(4000, sample_seconds / max(per, 1e-9))))
+    p.itmax = sweeps
+    t0 = time.perf_counter()
+    orc.poisson(p, f)
+    t_sweep = (time.perf_counter() - t0) / sweeps
+    return dict(t_transport=t_transport, t_sweep=t_sweep, sweeps=sweeps, threads=threads)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spl", type=int, default=0, help="SOR sweeps per launch (0 = library default)")
+    ap.add_argument("--tile-waves", type=int, default=0)
+    ap.add_argument("--wave-rows", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-dc", action="store_true", help="skip the FP32-refined leg")
+    ap.add_argument("--sor-launches", type=int, default=400)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the single cavity would "
+              "shard by row strips with halo exchange; see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cnv
+    C = N * N
+    dev = cnv.device_name(0)
+    print("bench: %s, %dx%d Re=%g, %d warmup + %d timed steps" % (dev, N, N, RE, args.warmup, args.steps),
+          file=sys.stderr, flush=True)
+
+    # ---- FP64 plain ---------------------------------------------------------------------
+    cav = cnv.Cavity(workload_params(cnv, cnv.PREC_FP64, args))
+    if args.warmup:
+        timed_steps(cav, args.warmup, "fp64 warmup")
+    st, ms = timed_steps(cav, args.steps, "fp64")
+    sweeps = [s.sweeps for s in st]
+    T = ms / 1e3
+    value = C * args.steps / T / 1e6
+    alg_bytes = sum(24 + BYTES_PER_CELL_SWEEP * k for k in sweeps) * C
+    state64 = cav.get_state()
+    spl = cav.sweeps_per_launch
+    print("bench: fp64 %.3f ms/step, sweeps/step %s" % (ms / args.steps, sweeps), file=sys.stderr, flush=True)
+    sor_ms = cav.time_sor(args.sor_launches)
+    cav.close()
+    achieved = spl * BYTES_PER_CELL_SWEEP * C / (sor_ms / 1e3)
+
+    # ---- FP32 + FP64 defect correction ------------------------------------------------------
+    dc = None
+    if not args.no_dc:
+        cd = cnv.Cavity(workload_params(cnv, cnv.PREC_FP32_DC, args))
+        if args.warmup:
+            timed_steps(cd, args.warmup, "dc warmup")
+        st32, ms32 = timed_steps(cd, args.steps, "dc")
+        s32 = cd.get_state()
+        sor32_ms = cd.time_sor(args.sor_launches)
+        cd.close()
+        mid = N // 2
+        def rel(a, b):
+            return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        def rel2(a, b):
+            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        dc = {
+            "value": C * args.steps / (ms32 / 1e3) / 1e6,
+            "ms_per_step": ms32 / args.steps,
+            "speedup_vs_fp64": ms / ms32,
+            "inner_sweeps_per_step": [s.sweeps for s in st32],
+            "outer_per_step": [s.outer for s in st32],
+            "err_linf_rel": {"psi": rel(s32["psi"], state64["psi"]), "omega": rel(s32["w"], state64["w"]),
+                             "u_centerline": rel(s32["u"][:, mid], state64["u"][:, mid]),
+                             "v_centerline": rel(s32["v"][mid, :], state64["v"][mid, :])},
+            "err_l2_rel": {"psi": rel2(s32["psi"], state64["psi"]), "omega": rel2(s32["w"], state64["w"]),
+                           "u_centerline": rel2(s32["u"][:, mid], state64["u"][:, mid]),
+                           "v_centerline": rel2(s32["v"][mid, :], state64["v"][mid, :])},
+            "sor_fp32_launch_ms": sor32_ms,
+            "sor_fp32_achieved_GBps": spl * BYTES_PER_CELL_SWEEP_FP32 * C / (sor32_ms / 1e3) / 1e9,
+        }
+        print("bench: dc %.3f ms/step, inner sweeps %s" % (ms32 / args.steps, dc["inner_sweeps_per_step"]),
+              file=sys.stderr, flush=True)
+
+    # ---- CPU baseline -----------------------------------------------------------------------
+    cpu = None
+    if not args.no_cpu:
+        b = cpu_baseline()
+        kmean = float(np.mean(sweeps))
+        t_step = b["t_transport"] + kmean * b["t_sweep"]
+        cpu = {"value": C / t_step / 1e6, "unit": "Mcell-steps/s", "cores": b["threads"], "kind": "port",
+               "sample": "oracle (C restatement, OpenMP red-black, %d threads) at 1024^2: step-0 transport "
+                         "(%.3f s) + %d SOR sweeps (%.2f ms/sweep), scaled by the GPU run's mean %.0f sweeps/step"
+                         % (b["threads"], b["t_transport"], b["sweeps"], b["t_sweep"] * 1e3, kmean)}
+
+    traffic = read_traffic("sor_fp64_n1024_traffic.txt")
+    line = {
+        "metric": "Mcell-steps/s & %HBM peak, 1024^2 Re1000 cavity; FP32-refined vs FP64 plain",
+        "value": value,
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (deterministic lid-driven cavity from rest)",
+        "config": {"workload": "cavity 1024x1024 Re1000 FP64 red-black SOR omega=1.95 tol=1e-10, order 6",
+                   "n": N, "Re": RE, "dt": DT, "sor_omega": OMEGA, "tol": TOL, "order": 6,
+                   "sweeps_per_launch": spl, "sweeps_per_step": sweeps,
+                   "algorithmic_GBps_step_loop": alg_bytes / T / 1e9},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "kernel": "sor_rb_kernel<double,...,S=%d>" % spl, "launch_ms": sor_ms},
+        "cpu_baseline": cpu,
+        "fp32_refined": dc,
+        "device": dev,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -9,23 +9,27 @@ namespace cnv {
 constexpr int kStencilWidth = 7;  // every Diff1/Diff2 row fits a 7-column window
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
 
-// Device-side control block of one Poisson solve.  Written only by the last workgroup of
-// each SOR launch (and by the init kernel); read by every workgroup at launch start.
-enum SorStatus : int { SOR_RUNNING = 0, SOR_CONVERGED = 1, SOR_FAILED = 2, SOR_ROLLBACK = 3 };
+// Device-side control block of one Poisson solve.  Written only by the decide kernel (after
+// each SOR launch) and by the init kernels; read by every SOR workgroup at launch start.
+enum SorStatus : int {
+    SOR_RUNNING = 0,
+    SOR_CONVERGED = 1,  // a sweep's L1 update fell below tol
+    SOR_FAILED = 2,     // itmax sweeps without convergence
+    SOR_ROLLBACK = 3,   // redo s_rerun sweeps from the unchanged source buffer, then converge
+    SOR_FOLD = 4        // defect correction: the FP32 iterate reached its resolution (rtol)
+};
 
 struct SorCtl {
     int k;              // sweeps accepted so far
-    int parity;         // which psi buffer holds the current iterate
+    int parity;         // which buffer holds the current iterate
     int status;         // SorStatus
     int s_rerun;        // after a rollback: sweeps to redo from the unchanged source buffer
-    unsigned ticket;    // last-workgroup election counter, reset by the last workgroup
     int launches;       // launches that did work (diagnostic)
-    int rel_first;      // 1: threshold becomes rtol * (first sweep's update), then 0
     int itmax;          // sweep budget of this solve
     double e_last;      // L1 update of the last accepted sweep
-    double tol;         // current stopping threshold on the L1 update
-    double rtol;        // relative inner tolerance (defect correction)
-    double tol_floor;   // lower bound on the threshold (defect correction)
+    double tol;         // stop when a sweep's L1 update < tol
+    double rtol;        // fold when the L1 update < rtol * L1 norm of the iterate (0: never)
+    double n_last;      // L1 norm of the iterate after the last accepted sweep (FP32 planes)
 };
 
 // Continuity diagnostic: max/min as order-preserving unsigned keys (exact, order independent).
@@ -69,15 +73,23 @@ struct SorArgs {
     const T* src_f;      // source term: w (plain path, negate = 1) or the FP32 residual
     T* psi0;
     T* psi1;
-    double* partial;     // [S][ntiles] per-workgroup L1 updates
+    double* partial;     // [S][ntiles][2]: per-workgroup L1 update and iterate norm
     SorCtl* ctl;
     int nr, nc, ld, tiles_i, tiles_j;  // rows (i, y) and columns (j, x) of the grid
     int negate;          // 1: f = -src (the reference's invsig before the solve)
     T c1, c2, c3, B, beta, omb;  // dy*dy, dx*dx, dx*dx*dy*dy, 2*(dx*dx+dy*dy), beta, 1-beta
+    T rB;                // RN(1/B)
 };
 
 // ---- host launch wrappers (defined in the .hip files) ---------------------------------
-struct TileShape { int ti, tj, s; };
+// SOR launch shape: s sweeps per launch, nw waves per workgroup holding rb rows each; the
+// workgroup's tile owns (nw*rb - 4s) rows x (128 - 4s) columns.
+struct TileShape { int s, nw, rb; };
+constexpr int kMaxSweepsPerLaunch = 8;
+inline int tile_cols(const TileShape& t) { return 128 - 4 * t.s; }
+inline int tile_rows(const TileShape& t) { return t.nw * t.rb - 4 * t.s; }
+
+int sor_decide_launch(SorCtl* ctl, const double* partial, int ntiles, int S, hipStream_t st);
 
 int sor_fp64_launch(const SorArgs<double>& a, TileShape shape, bool gs, hipStream_t st);
 int sor_fp32_launch(const SorArgs<float>& a, TileShape shape, bool gs, hipStream_t st);
@@ -110,7 +122,7 @@ int dc_residual_launch(const double* w, const double* psi, float* r32, Diag* dia
                        hipStream_t st);
 int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCtl* ctl, int n,
                       int ld, hipStream_t st);
-int dc_inner_init_launch(float* e0, SorCtl* ctl, int n, int ld, double rtol, double floor_,
+int dc_inner_init_launch(float* e0, SorCtl* ctl, int n, int ld, double rtol, double tol,
                          int itmax, hipStream_t st);
 int zero_f32_launch(float* p, long long count, hipStream_t st);
 int velocity_fp32_launch(const double* psi, double* u, double* v, int n, int ld, const Stencils& s,

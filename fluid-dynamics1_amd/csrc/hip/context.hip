@@ -61,7 +61,7 @@ struct cnv_ctx {
     float* e32[2] = {nullptr, nullptr};
     float* r32 = nullptr;
     // launch shape
-    TileShape shape{32, 64, 4};
+    TileShape shape{8, 8, 10};
     int tiles_i = 0, tiles_j = 0;
     int launches_per_poll = 32;
     int k_prev = 0;       // sweeps of the previous solve (queue-ahead prediction)
@@ -139,12 +139,14 @@ int cnv_device_name(int device, char* buf, int len) {
     return CNV_OK;
 }
 
+// Default launch shape per sweeps-per-launch (tuned on MI355X at 1024^2).
 static TileShape shape_for(int spl) {
     switch (spl) {
-        case 1: return TileShape{32, 64, 1};
-        case 2: return TileShape{32, 64, 2};
-        case 8: return TileShape{32, 64, 8};
-        default: return TileShape{32, 64, 4};
+        case 1: return TileShape{1, 4, 10};
+        case 2: return TileShape{2, 4, 10};
+        case 4: return TileShape{4, 8, 8};
+        case 6: return TileShape{6, 8, 8};
+        default: return TileShape{8, 8, 10};
     }
 }
 
@@ -287,15 +289,18 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
     c->ld = p->n;
     c->cells = (size_t)p->n * p->n;
     c->bc = WallBC{p->u1, p->u2, p->u3, p->u4, p->v1, p->v2, p->v3, p->v4};
-    int spl = p->sweeps_per_launch ? p->sweeps_per_launch : 4;
+    int spl = p->sweeps_per_launch ? p->sweeps_per_launch : 8;
     c->shape = shape_for(spl);
-    if (p->sweeps_per_launch && c->shape.s != p->sweeps_per_launch) {
+    if (p->tile_waves > 0) c->shape.nw = p->tile_waves;
+    if (p->wave_rows > 0) c->shape.rb = p->wave_rows;
+    if ((p->sweeps_per_launch && c->shape.s != p->sweeps_per_launch) || !sor_shape_supported(c->shape)) {
         cnv_destroy(c);
-        return fail(nullptr, CNV_ERR_ARG, "sweeps_per_launch must be 1, 2, 4 or 8");
+        return fail(nullptr, CNV_ERR_ARG, "unsupported SOR launch shape (sweeps_per_launch %d, waves %d, rows per wave %d)",
+                    spl, p->tile_waves, p->wave_rows);
     }
     c->launches_per_poll = p->launches_per_poll > 0 ? p->launches_per_poll : 32;
-    c->tiles_i = (c->n + c->shape.ti - 1) / c->shape.ti;
-    c->tiles_j = (c->n + c->shape.tj - 1) / c->shape.tj;
+    c->tiles_i = (c->n + tile_rows(c->shape) - 1) / tile_rows(c->shape);
+    c->tiles_j = (c->n + tile_cols(c->shape) - 1) / tile_cols(c->shape);
     auto setup = [&]() -> int {
         HIPC(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         const size_t bytes = sizeof(double) * c->cells;
@@ -309,7 +314,7 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
         HIPC(c, hipMalloc(&c->diag, sizeof(Diag)));
         HIPC(c, hipMemset(c->ctl, 0, sizeof(SorCtl)));
         HIPC(c, hipMemset(c->diag, 0, sizeof(Diag)));
-        HIPC(c, hipMalloc(&c->partial, sizeof(double) * 8 * (size_t)c->tiles_i * c->tiles_j));
+        HIPC(c, hipMalloc(&c->partial, sizeof(double) * 2 * kMaxSweepsPerLaunch * (size_t)c->tiles_i * c->tiles_j));
         HIPC(c, hipHostMalloc(&c->h_ctl, sizeof(SorCtl) * kSlots, hipHostMallocDefault));
         HIPC(c, hipHostMalloc(&c->h_diag, sizeof(Diag), hipHostMallocDefault));
         for (int s = 0; s < kSlots; ++s) HIPC(c, hipEventCreateWithFlags(&c->ev[s], hipEventDisableTiming));
@@ -360,13 +365,22 @@ static SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int negate) {
     a.B = (T)(2 * (dx * dx + dy * dy));
     a.beta = (T)c->p.beta;
     a.omb = (T)(1 - c->p.beta);
+    a.rB = (T)(1.0 / (2 * (dx * dx + dy * dy)));
     return a;
 }
 
 // Queue SOR launches until the device control block reports convergence or failure.  Keeps
 // one chunk queued behind the one being waited on, so the GPU never idles on the host.
-static int launch_sor(cnv_ctx* c, const SorArgs<double>& a, bool gs) { return sor_fp64_launch(a, c->shape, gs, c->stream); }
-static int launch_sor(cnv_ctx* c, const SorArgs<float>& a, bool gs) { return sor_fp32_launch(a, c->shape, gs, c->stream); }
+static int launch_sor(cnv_ctx* c, const SorArgs<double>& a, bool gs) {
+    if (sor_fp64_launch(a, c->shape, gs, c->stream)) return 1;
+    return sor_decide_launch(c->ctl, c->partial, c->tiles_i * c->tiles_j, c->shape.s, c->stream);
+}
+static int launch_sor(cnv_ctx* c, const SorArgs<float>& a, bool gs) {
+    if (sor_fp32_launch(a, c->shape, gs, c->stream)) return 1;
+    return sor_decide_launch(c->ctl, c->partial, c->tiles_i * c->tiles_j, c->shape.s, c->stream);
+}
+
+static bool sor_done(int status) { return status != SOR_RUNNING && status != SOR_ROLLBACK; }
 
 template <typename T>
 static int run_sor(cnv_ctx* c, const SorArgs<T>& a, SorCtl* final_ctl) {
@@ -399,7 +413,7 @@ static int run_sor(cnv_ctx* c, const SorArgs<T>& a, SorCtl* final_ctl) {
             --npend;
             HIPC(c, hipEventSynchronize(c->ev[s0]));
             const SorCtl snap = c->h_ctl[s0];
-            if (snap.status == SOR_CONVERGED || snap.status == SOR_FAILED) {
+            if (sor_done(snap.status)) {
                 *final_ctl = snap;
                 return CNV_OK;
             }
@@ -418,18 +432,19 @@ static int poisson_fp64(cnv_ctx* c, const double* f_src, int negate, SorCtl* fin
 }
 
 // defect correction: psi[0] accumulates in FP64, e32 ping-pong holds the inner FP32 iterates
-static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer, int* inner_sweeps) {
+static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) {
     LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, c->p.tol, c->p.itmax, c->stream));
     const double dx = c->p.dx, dy = c->p.dy;
     const double c1 = dy * dy, c2 = dx * dx, c3 = dx * dx * dy * dy, B = 2 * (dx * dx + dy * dy);
     SorArgs<float> a = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
     int total = 0;
     const int max_outer = c->p.max_outer > 0 ? c->p.max_outer : 50;
-    for (int o = 0; o < max_outer; ++o) {
+    *fin = SorCtl{};
+    fin->status = SOR_FAILED;
+    for (int o = 0; o < max_outer && total < c->p.itmax; ++o) {
         LAUNCH(c, dc_residual_launch(w_src, c->psi[0], c->r32, c->diag, c->n, c->ld, c1, c2, c3, B, c->p.beta, c->stream));
-        const int budget = c->p.itmax - total;
-        LAUNCH(c, dc_inner_init_launch(c->e32[0], c->ctl, c->n, c->ld, c->p.inner_rtol, c->p.tol, budget, c->stream));
-        // e32[1] is written in full by the first launch; its edges must read as zero
+        LAUNCH(c, dc_inner_init_launch(c->e32[0], c->ctl, c->n, c->ld, c->p.inner_rtol, c->p.tol, c->p.itmax - total,
+                                       c->stream));
         SorCtl snap{};
         int rc = run_sor<float>(c, a, &snap);
         if (rc) return rc;
@@ -438,13 +453,8 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer, 
         *outer = o + 1;
         *fin = snap;
         fin->k = total;
-        if (snap.status == SOR_FAILED) return CNV_OK;
-        if (snap.e_last < c->p.tol) {
-            *inner_sweeps = total;
-            return CNV_OK;
-        }
+        if (snap.status == SOR_CONVERGED || snap.status == SOR_FAILED) return CNV_OK;
     }
-    *inner_sweeps = total;
     fin->status = SOR_FAILED;
     return CNV_OK;
 }
@@ -462,11 +472,13 @@ static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
         LAUNCH(c, transport_fp64_launch(w_in, w_out, c->u, c->v, n, ld, c->st, c->bc, rRe, c->p.dt, c->stream));
     c->wcur ^= 1;
     SorCtl fin{};
-    int outer = 0, inner = 0;
-    int rc = dc ? poisson_dc(c, w_out, &fin, &outer, &inner) : poisson_fp64(c, w_out, 1, &fin);
+    int outer = 0;
+    int rc = dc ? poisson_dc(c, w_out, &fin, &outer) : poisson_fp64(c, w_out, 1, &fin);
     if (rc) return rc;
+    // velocities in FP64 in both forms: v = -(DX psi) takes small differences of a large psi,
+    // which FP32 stencils would resolve only to ~1e-7 of |psi|/h
     if (dc)
-        LAUNCH(c, velocity_fp32_launch(c->psi[0], c->u, c->v, n, ld, c->st, c->stream));
+        LAUNCH(c, velocity_launch(c->psi[0], c->psi[0], c->ctl, c->u, c->v, n, ld, c->st, c->stream));
     else
         LAUNCH(c, velocity_launch(c->psi[0], c->psi[1], c->ctl, c->u, c->v, n, ld, c->st, c->stream));
     if (c->p.diagnostics) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, n, ld, c->st, c->stream));
@@ -564,7 +576,8 @@ int cnv_time_sor(cnv_ctx* c, int launches, double* mean_ms) {
     SorArgs<double> a64 = sor_args<double>(c, c->w[c->wcur], c->psi[0], c->psi[1], 1);
     SorArgs<float> a32 = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
     const bool gs = c->p.poisson_type == 1;
-    // warm the instruction cache and clocks, then time
+    // warm the instruction cache and clocks, then time the SOR kernel alone (without the
+    // one-workgroup decide launch the solver queues after each SOR launch)
     for (int l = 0; l < 8; ++l)
         LAUNCH(c, dc ? sor_fp32_launch(a32, c->shape, gs, c->stream) : sor_fp64_launch(a64, c->shape, gs, c->stream));
     HIPC(c, hipEventRecord(c->t0, c->stream));

@@ -2,17 +2,19 @@
 //
 // Poisson solve as defect correction around the same SOR iteration as the plain path:
 //   psi64 = 0
-//   repeat:  r   = f - L psi64                       (FP64 residual, rounded to FP32)
-//            e   = SOR_fp32(L e = r) from e = 0        (same red-black kernel, FP32 planes)
-//            psi64 += e                               (FP64 accumulation)
-//   until the inner SOR stopped on the absolute criterion (its last sweep's L1 update < tol).
+//   repeat:  r   = f - L psi64                         (FP64 residual, rounded to FP32)
+//            e   = SOR_fp32(L e = r) from e = 0         (same red-black kernel on FP32 planes)
+//            psi64 += e                                 (FP64 accumulation)
+//   until an inner sweep's L1 update fell below tol (the plain solver's own stopping rule).
 // L is the operator whose fixed point the reference SOR iterates to (src/poisson.c:268):
 //   L psi = (dy*dy*(N+S) + dx*dx*(E+W) - 2*(dx*dx+dy*dy)*psi) / (dx*dx*dy*dy).
-// Running SOR on L e = r from zero produces the iterates that SOR on psi would produce from
-// psi64, so the inner sweep's update norm is the plain solver's stopping quantity; each inner
-// solve also stops once its update fell by inner_rtol (FP32 cannot resolve more), after which
-// the residual is refreshed in FP64.  Only the residual, the psi accumulation and the omega update
-// are FP64; the transport and velocity stencils run in FP32.
+// SOR is a one-step stationary iteration, so SOR on L e = r from e = 0 produces exactly the
+// iterates that SOR on psi would produce from psi64: the inner sweeps continue the plain
+// solver's iteration and their update norm is its stopping quantity.  An inner solve ends early
+// ("fold") once its update norm fell below inner_rtol times the L1 norm of e, where FP32 stops
+// resolving the update; the correction is then folded into psi64 and the residual refreshed in
+// FP64, which restores full resolution.  Only the residual, the psi accumulation and the omega
+// update are FP64; the transport and velocity stencils run in FP32.
 #include <float.h>
 
 #include "cnv_device.h"
@@ -24,8 +26,7 @@ int sor_fp32_launch(const SorArgs<float>& a, TileShape shape, bool gs, hipStream
     return sor_dispatch<float>(a, shape, gs, st);
 }
 
-// r32 = (float)(f - L psi) on the interior, f = -w; 0 on the edges.  Also the L1 norm of the
-// Jacobi-scaled residual (beta*c3*r/B), summed in a fixed order through the tickets of Diag.
+// r32 = (float)(f - L psi) on the interior, f = -w; 0 on the edges.
 __global__ void __launch_bounds__(kBlock) dc_residual_kernel(const double* __restrict__ w,
                                                             const double* __restrict__ psi,
                                                             float* __restrict__ r32, int n, int ld,
@@ -52,9 +53,10 @@ __global__ void __launch_bounds__(kBlock) dc_correct_kernel(double* __restrict__
     if (q < count) psi[q] += (double)e[q];
 }
 
-// start of an inner solve: e = 0, control block reset with the relative first-sweep rule
+// start of an inner solve: e = 0, control block reset with the absolute tolerance (the plain
+// solver's stopping rule) and the FP32 fold ratio
 __global__ void __launch_bounds__(kBlock) dc_inner_init_kernel(float* e0, SorCtl* ctl, long long count,
-                                                              double rtol, double floor_, int itmax) {
+                                                              double rtol, double tol, int itmax) {
     const long long stride = (long long)gridDim.x * kBlock;
     for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) e0[q] = 0.0f;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -62,14 +64,12 @@ __global__ void __launch_bounds__(kBlock) dc_inner_init_kernel(float* e0, SorCtl
         ctl->parity = 0;
         ctl->status = itmax > 0 ? SOR_RUNNING : SOR_FAILED;
         ctl->s_rerun = 0;
-        ctl->ticket = 0u;
         ctl->launches = 0;
-        ctl->rel_first = 1;
         ctl->itmax = itmax;
         ctl->e_last = 0.0;
-        ctl->tol = floor_;
+        ctl->tol = tol;
         ctl->rtol = rtol;
-        ctl->tol_floor = floor_;
+        ctl->n_last = 0.0;
     }
 }
 
@@ -154,12 +154,12 @@ int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCt
     return launched();
 }
 
-int dc_inner_init_launch(float* e0, SorCtl* ctl, int n, int ld, double rtol, double floor_, int itmax,
+int dc_inner_init_launch(float* e0, SorCtl* ctl, int n, int ld, double rtol, double tol, int itmax,
                          hipStream_t st) {
     const long long count = (long long)n * ld;
     unsigned nb = blocks_for(count);
     if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(dc_inner_init_kernel, dim3(nb), dim3(kBlock), 0, st, e0, ctl, count, rtol, floor_, itmax);
+    hipLaunchKernelGGL(dc_inner_init_kernel, dim3(nb), dim3(kBlock), 0, st, e0, ctl, count, rtol, tol, itmax);
     return launched();
 }
 

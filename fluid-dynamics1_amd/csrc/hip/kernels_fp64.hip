@@ -103,14 +103,12 @@ __global__ void __launch_bounds__(kBlock) poisson_init_kernel(double* psi0, SorC
         ctl->parity = 0;
         ctl->status = itmax > 0 ? SOR_RUNNING : SOR_FAILED;
         ctl->s_rerun = 0;
-        ctl->ticket = 0u;
         ctl->launches = 0;
-        ctl->rel_first = 0;
         ctl->itmax = itmax;
         ctl->e_last = 0.0;
         ctl->tol = tol;
         ctl->rtol = 0.0;
-        ctl->tol_floor = 0.0;
+        ctl->n_last = 0.0;
         if (diag) {
             diag->kmax = dkey(-DBL_MAX);
             diag->kmin = dkey(DBL_MAX);
@@ -165,6 +163,81 @@ __global__ void __launch_bounds__(kBlock) continuity_kernel(const double* __rest
     }
 }
 
+// ---- SOR decision: reduce the launch's per-tile partials in tile order, update the control
+//      block (convergence test of src/poisson.c:272-278 per sweep, rollback, budget, fold) ----
+__global__ void __launch_bounds__(kBlock) sor_decide_kernel(SorCtl* ctl, const double* __restrict__ partial,
+                                                           int ntiles, int S) {
+    constexpr int NW = kBlock / 64;
+    __shared__ double red[NW][kMaxSweepsPerLaunch][2];
+    const int status = ctl->status;
+    if (status != SOR_RUNNING && status != SOR_ROLLBACK) return;
+    const int k0 = ctl->k;
+    const int budget = ctl->itmax - k0;
+    const int s_eff = (status == SOR_ROLLBACK) ? ctl->s_rerun : (budget < S ? budget : S);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // one pass: every thread loads all sweeps of its tiles at once, then one reduction
+    double e[kMaxSweepsPerLaunch], nn[kMaxSweepsPerLaunch];
+#pragma unroll
+    for (int s = 0; s < kMaxSweepsPerLaunch; ++s) e[s] = nn[s] = 0.0;
+    for (int t = tid; t < ntiles; t += kBlock) {
+#pragma unroll
+        for (int s = 0; s < kMaxSweepsPerLaunch; ++s) {
+            if (s < s_eff) {
+                e[s] += partial[((size_t)s * ntiles + t) * 2];
+                nn[s] += partial[((size_t)s * ntiles + t) * 2 + 1];
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kMaxSweepsPerLaunch; ++s) {
+        const double a = wave_sum(e[s]), b = wave_sum(nn[s]);
+        if (lane == 0) {
+            red[wv][s][0] = a;
+            red[wv][s][1] = b;
+        }
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    double etot[kMaxSweepsPerLaunch], ntot[kMaxSweepsPerLaunch];
+    for (int s = 0; s < s_eff; ++s) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < NW; ++w) {
+            a += red[w][s][0];
+            b += red[w][s][1];
+        }
+        etot[s] = a;
+        ntot[s] = b;
+    }
+    const double thr = ctl->tol, fold = ctl->rtol;
+    int s_conv = -1, s_fold = -1;
+    for (int s = 0; s < s_eff; ++s) {
+        if (etot[s] < thr) {
+            s_conv = s;
+            break;
+        }
+        if (fold > 0.0 && etot[s] < fold * ntot[s]) {
+            s_fold = s;
+            break;
+        }
+    }
+    if (s_conv >= 0 && s_conv < s_eff - 1) {
+        ctl->status = SOR_ROLLBACK;
+        ctl->s_rerun = s_conv + 1;
+    } else {
+        ctl->k = k0 + s_eff;
+        ctl->parity ^= 1;
+        ctl->e_last = etot[s_eff - 1];
+        ctl->n_last = ntot[s_eff - 1];
+        if (s_conv >= 0)
+            ctl->status = SOR_CONVERGED;
+        else if (s_fold >= 0)
+            ctl->status = SOR_FOLD;
+        else
+            ctl->status = (k0 + s_eff >= ctl->itmax) ? SOR_FAILED : SOR_RUNNING;
+    }
+    ctl->launches += 1;
+}
+
 // ---- legacy elementwise entry points -------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) euler_kernel(double* w, const double* dwdx, const double* dwdy,
                                                       const double* d2x, const double* d2y,
@@ -193,8 +266,13 @@ int sor_fp64_launch(const SorArgs<double>& a, TileShape shape, bool gs, hipStrea
     return sor_dispatch<double>(a, shape, gs, st);
 }
 
+int sor_decide_launch(SorCtl* ctl, const double* partial, int ntiles, int S, hipStream_t st) {
+    hipLaunchKernelGGL(sor_decide_kernel, dim3(1), dim3(kBlock), 0, st, ctl, partial, ntiles, S);
+    return launched();
+}
+
 bool sor_shape_supported(TileShape sh) {
-#define CNV_SOR_OK(ti_, tj_, s_) if (sh.ti == ti_ && sh.tj == tj_ && sh.s == s_) return true;
+#define CNV_SOR_OK(s_, nw_, rb_) if (sh.s == s_ && sh.nw == nw_ && sh.rb == rb_) return true;
     CNV_SOR_SHAPES(CNV_SOR_OK)
 #undef CNV_SOR_OK
     return false;

@@ -50,7 +50,9 @@ typedef struct {
     double inner_rtol;     /* inner FP32 solve: stop at this fraction of its first update    */
     int max_outer;         /* defect-correction iterations allowed per solve                */
     /* tuning (0 = automatic) */
-    int sweeps_per_launch; /* SOR sweeps fused per kernel launch (1, 2, 4 or 8)              */
+    int sweeps_per_launch; /* SOR sweeps fused per kernel launch (1, 2, 4, 6 or 8)           */
+    int tile_waves;        /* wave64s per SOR workgroup (compiled shapes only)               */
+    int wave_rows;         /* grid rows held in registers per wave (compiled shapes only)    */
     int launches_per_poll; /* SOR launches queued between host polls of the convergence flag */
     int diagnostics;       /* 1 = continuity max/min every step (src/main.c:387-408)        */
 } cnv_params;
