@@ -9,21 +9,21 @@ namespace cnv {
 constexpr int kStencilWidth = 7;  // every Diff1/Diff2 row fits a 7-column window
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
 
-// Device-side control block of one Poisson solve.  Written only by the decide kernel (after
-// each SOR launch) and by the init kernels; read by every SOR workgroup at launch start.
+// Device-side state of one Poisson solve, kept in two slots: SOR launch L (launch index passed
+// by the host, counted from 0 per solve) reads slot (L-1)&1, decides on the previous launch's
+// per-tile partial sums, and writes slot L&1.  The init kernels write slot 1 for launch 0.
 enum SorStatus : int {
     SOR_RUNNING = 0,
     SOR_CONVERGED = 1,  // a sweep's L1 update fell below tol
     SOR_FAILED = 2,     // itmax sweeps without convergence
-    SOR_ROLLBACK = 3,   // redo s_rerun sweeps from the unchanged source buffer, then converge
     SOR_FOLD = 4        // defect correction: the FP32 iterate reached its resolution (rtol)
 };
 
 struct SorCtl {
-    int k;              // sweeps accepted so far
-    int parity;         // which buffer holds the current iterate
-    int status;         // SorStatus
-    int s_rerun;        // after a rollback: sweeps to redo from the unchanged source buffer
+    int k;              // sweeps accepted before the launch that wrote the slot
+    int parity;         // terminal states: the buffer holding the final iterate
+    int status;         // SorStatus after deciding on the previous launch
+    int s_done;         // sweeps the writing launch performs itself (0: none)
     int launches;       // launches that did work (diagnostic)
     int itmax;          // sweep budget of this solve
     double e_last;      // L1 update of the last accepted sweep
@@ -73,8 +73,10 @@ struct SorArgs {
     const T* src_f;      // source term: w (plain path, negate = 1) or the FP32 residual
     T* psi0;
     T* psi1;
-    double* partial;     // [S][ntiles][2]: per-workgroup L1 update and iterate norm
-    SorCtl* ctl;
+    double* partial;     // [2][kMaxSweepsPerLaunch][ntiles][2]: per-workgroup L1 update and
+                         // iterate norm, double-buffered by launch parity
+    SorCtl* ctl;         // the solve's two state slots
+    int launch;          // launch index within the solve
     int nr, nc, ld, tiles_i, tiles_j;  // rows (i, y) and columns (j, x) of the grid
     int negate;          // 1: f = -src (the reference's invsig before the solve)
     T c1, c2, c3, B, beta, omb;  // dy*dy, dx*dx, dx*dx*dy*dy, 2*(dx*dx+dy*dy), beta, 1-beta
@@ -89,7 +91,7 @@ constexpr int kMaxSweepsPerLaunch = 8;
 inline int tile_cols(const TileShape& t) { return 128 - 4 * t.s; }
 inline int tile_rows(const TileShape& t) { return t.nw * t.rb - 4 * t.s; }
 
-int sor_decide_launch(SorCtl* ctl, const double* partial, int ntiles, int S, hipStream_t st);
+
 
 int sor_fp64_launch(const SorArgs<double>& a, TileShape shape, bool gs, hipStream_t st);
 int sor_fp32_launch(const SorArgs<float>& a, TileShape shape, bool gs, hipStream_t st);
@@ -105,7 +107,7 @@ int transport_fp32_launch(const double* w_in, double* w_out, const double* u, co
                           double dt, hipStream_t st);
 int poisson_init_launch(double* psi0, SorCtl* ctl, Diag* diag, int n, int ld, double tol,
                         int itmax, hipStream_t st);
-int velocity_launch(const double* psi0, const double* psi1, const SorCtl* ctl, double* u,
+int velocity_launch(const double* psi0, const double* psi1, const SorCtl* fin, double* u,
                     double* v, int n, int ld, const Stencils& s, hipStream_t st);
 int continuity_launch(const double* u, const double* v, Diag* diag, int n, int ld,
                       const Stencils& s, hipStream_t st);
@@ -120,7 +122,7 @@ int copy_launch(double* dst, const double* src, long long count, hipStream_t st)
 int dc_residual_launch(const double* w, const double* psi, float* r32, Diag* diag, int n,
                        int ld, double c1, double c2, double c3, double B, double beta,
                        hipStream_t st);
-int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCtl* ctl, int n,
+int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCtl* fin, int n,
                       int ld, hipStream_t st);
 int dc_inner_init_launch(float* e0, SorCtl* ctl, int n, int ld, double rtol, double tol,
                          int itmax, hipStream_t st);

@@ -50,7 +50,8 @@ struct cnv_ctx {
     float* d_ftab = nullptr;
     Stencils st{};
     // solver control
-    SorCtl* ctl = nullptr;
+    SorCtl* ctl = nullptr;    // the Poisson solve's two state slots
+    int fin_slot = 0;         // slot written by the last SOR launch of the last solve
     Diag* diag = nullptr;
     double* partial = nullptr;
     SorCtl* h_ctl = nullptr;  // pinned, kSlots entries
@@ -204,9 +205,9 @@ static int current_psi(cnv_ctx* c, const double** out) {
         *out = c->psi[0];
         return CNV_OK;
     }
-    HIPC(c, hipMemcpyAsync(c->h_ctl, c->ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(c->h_ctl, c->ctl + c->fin_slot, sizeof(SorCtl), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
-    *out = c->psi[c->h_ctl[0].parity & 1];
+    *out = c->psi[c->h_ctl[0].parity & 1];  // parity of the final iterate
     return CNV_OK;
 }
 
@@ -310,11 +311,11 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
         }
         HIPC(c, hipMalloc(&c->u, bytes));
         HIPC(c, hipMalloc(&c->v, bytes));
-        HIPC(c, hipMalloc(&c->ctl, sizeof(SorCtl)));
+        HIPC(c, hipMalloc(&c->ctl, 2 * sizeof(SorCtl)));
         HIPC(c, hipMalloc(&c->diag, sizeof(Diag)));
-        HIPC(c, hipMemset(c->ctl, 0, sizeof(SorCtl)));
+        HIPC(c, hipMemset(c->ctl, 0, 2 * sizeof(SorCtl)));
         HIPC(c, hipMemset(c->diag, 0, sizeof(Diag)));
-        HIPC(c, hipMalloc(&c->partial, sizeof(double) * 2 * kMaxSweepsPerLaunch * (size_t)c->tiles_i * c->tiles_j));
+        HIPC(c, hipMalloc(&c->partial, sizeof(double) * 2 * 2 * kMaxSweepsPerLaunch * (size_t)c->tiles_i * c->tiles_j));
         HIPC(c, hipHostMalloc(&c->h_ctl, sizeof(SorCtl) * kSlots, hipHostMallocDefault));
         HIPC(c, hipHostMalloc(&c->h_diag, sizeof(Diag), hipHostMallocDefault));
         for (int s = 0; s < kSlots; ++s) HIPC(c, hipEventCreateWithFlags(&c->ev[s], hipEventDisableTiming));
@@ -371,23 +372,26 @@ static SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int negate) {
 
 // Queue SOR launches until the device control block reports convergence or failure.  Keeps
 // one chunk queued behind the one being waited on, so the GPU never idles on the host.
-static int launch_sor(cnv_ctx* c, const SorArgs<double>& a, bool gs) {
-    if (sor_fp64_launch(a, c->shape, gs, c->stream)) return 1;
-    return sor_decide_launch(c->ctl, c->partial, c->tiles_i * c->tiles_j, c->shape.s, c->stream);
+static int launch_sor(cnv_ctx* c, SorArgs<double> a, int L, bool gs) {
+    a.launch = L;
+    return sor_fp64_launch(a, c->shape, gs, c->stream);
 }
-static int launch_sor(cnv_ctx* c, const SorArgs<float>& a, bool gs) {
-    if (sor_fp32_launch(a, c->shape, gs, c->stream)) return 1;
-    return sor_decide_launch(c->ctl, c->partial, c->tiles_i * c->tiles_j, c->shape.s, c->stream);
+static int launch_sor(cnv_ctx* c, SorArgs<float> a, int L, bool gs) {
+    a.launch = L;
+    return sor_fp32_launch(a, c->shape, gs, c->stream);
 }
 
-static bool sor_done(int status) { return status != SOR_RUNNING && status != SOR_ROLLBACK; }
+static bool sor_done(int status) { return status != SOR_RUNNING; }
 
+// Queue SOR launches (launch index L = 0, 1, ...) until the state slot written by a queued
+// launch reports a terminal status.  Keeps one chunk queued behind the one being waited on, so
+// the GPU never idles on the host; launches queued after the end are no-ops.
 template <typename T>
 static int run_sor(cnv_ctx* c, const SorArgs<T>& a, SorCtl* final_ctl) {
     const bool gs = (c->p.poisson_type == 1);
     const int S = c->shape.s;
     const int M = c->launches_per_poll;
-    const long long cap = (long long)c->p.itmax + 4LL * M + 8;  // working launches <= itmax/S + 1
+    const long long cap = (long long)c->p.itmax + 4LL * M + 8;  // working launches <= itmax/S + 2
     int first = M;
     if (c->k_prev > 0) {
         const int predicted = c->k_prev / S - M;
@@ -400,10 +404,12 @@ static int run_sor(cnv_ctx* c, const SorArgs<T>& a, SorCtl* final_ctl) {
     while (true) {
         const int chunk = queued == 0 ? first : M;
         for (int l = 0; l < chunk; ++l) {
-            if (launch_sor(c, a, gs)) return fail(c, CNV_ERR_HIP, "SOR launch failed: %s", hipGetErrorString(hipGetLastError()));
+            if (launch_sor(c, a, (int)(queued + l), gs))
+                return fail(c, CNV_ERR_HIP, "SOR launch failed: %s", hipGetErrorString(hipGetLastError()));
         }
         queued += chunk;
-        HIPC(c, hipMemcpyAsync(&c->h_ctl[slot], c->ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, c->stream));
+        c->fin_slot = (int)((queued - 1) & 1);
+        HIPC(c, hipMemcpyAsync(&c->h_ctl[slot], c->ctl + c->fin_slot, sizeof(SorCtl), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipEventRecord(c->ev[slot], c->stream));
         pending[npend++] = slot;
         slot = (slot + 1) % kSlots;
@@ -418,7 +424,7 @@ static int run_sor(cnv_ctx* c, const SorArgs<T>& a, SorCtl* final_ctl) {
                 return CNV_OK;
             }
         }
-        if (queued > cap + first) return fail(c, CNV_ERR_HIP, "SOR control block did not terminate after %lld launches", queued);
+        if (queued > cap + first) return fail(c, CNV_ERR_HIP, "SOR state did not terminate after %lld launches", queued);
     }
 }
 
@@ -448,7 +454,7 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) 
         SorCtl snap{};
         int rc = run_sor<float>(c, a, &snap);
         if (rc) return rc;
-        LAUNCH(c, dc_correct_launch(c->psi[0], c->e32[0], c->e32[1], c->ctl, c->n, c->ld, c->stream));
+        LAUNCH(c, dc_correct_launch(c->psi[0], c->e32[0], c->e32[1], c->ctl + c->fin_slot, c->n, c->ld, c->stream));
         total += snap.k;
         *outer = o + 1;
         *fin = snap;
@@ -478,9 +484,9 @@ static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
     // velocities in FP64 in both forms: v = -(DX psi) takes small differences of a large psi,
     // which FP32 stencils would resolve only to ~1e-7 of |psi|/h
     if (dc)
-        LAUNCH(c, velocity_launch(c->psi[0], c->psi[0], c->ctl, c->u, c->v, n, ld, c->st, c->stream));
+        LAUNCH(c, velocity_launch(c->psi[0], c->psi[0], c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
     else
-        LAUNCH(c, velocity_launch(c->psi[0], c->psi[1], c->ctl, c->u, c->v, n, ld, c->st, c->stream));
+        LAUNCH(c, velocity_launch(c->psi[0], c->psi[1], c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
     if (c->p.diagnostics) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, n, ld, c->st, c->stream));
     cnv_step_stats s{};
     s.status = fin.status == SOR_CONVERGED ? CNV_OK : CNV_NOT_CONVERGED;
@@ -576,13 +582,14 @@ int cnv_time_sor(cnv_ctx* c, int launches, double* mean_ms) {
     SorArgs<double> a64 = sor_args<double>(c, c->w[c->wcur], c->psi[0], c->psi[1], 1);
     SorArgs<float> a32 = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
     const bool gs = c->p.poisson_type == 1;
-    // warm the instruction cache and clocks, then time the SOR kernel alone (without the
-    // one-workgroup decide launch the solver queues after each SOR launch)
-    for (int l = 0; l < 8; ++l)
-        LAUNCH(c, dc ? sor_fp32_launch(a32, c->shape, gs, c->stream) : sor_fp64_launch(a64, c->shape, gs, c->stream));
+    // warm the instruction cache and clocks, then time back-to-back launches (each decides on
+    // its predecessor in its prologue, as in a solve)
+    int L = 0;
+    for (int l = 0; l < 8; ++l, ++L)
+        LAUNCH(c, dc ? launch_sor(c, a32, L, gs) : launch_sor(c, a64, L, gs));
     HIPC(c, hipEventRecord(c->t0, c->stream));
-    for (int l = 0; l < launches; ++l)
-        LAUNCH(c, dc ? sor_fp32_launch(a32, c->shape, gs, c->stream) : sor_fp64_launch(a64, c->shape, gs, c->stream));
+    for (int l = 0; l < launches; ++l, ++L)
+        LAUNCH(c, dc ? launch_sor(c, a32, L, gs) : launch_sor(c, a64, L, gs));
     HIPC(c, hipEventRecord(c->t1, c->stream));
     HIPC(c, hipEventSynchronize(c->t1));
     float ms = 0.f;

@@ -46,9 +46,9 @@ __global__ void __launch_bounds__(kBlock) dc_residual_kernel(const double* __res
 
 // psi64 += e32 (the inner solve's final iterate, chosen by the control block's parity)
 __global__ void __launch_bounds__(kBlock) dc_correct_kernel(double* __restrict__ psi, const float* e0,
-                                                           const float* e1, const SorCtl* ctl,
+                                                           const float* e1, const SorCtl* fin,
                                                            long long count) {
-    const float* e = ctl->parity ? e1 : e0;
+    const float* e = fin->parity ? e1 : e0;
     const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (q < count) psi[q] += (double)e[q];
 }
@@ -59,17 +59,18 @@ __global__ void __launch_bounds__(kBlock) dc_inner_init_kernel(float* e0, SorCtl
                                                               double rtol, double tol, int itmax) {
     const long long stride = (long long)gridDim.x * kBlock;
     for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) e0[q] = 0.0f;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->k = 0;
-        ctl->parity = 0;
-        ctl->status = itmax > 0 ? SOR_RUNNING : SOR_FAILED;
-        ctl->s_rerun = 0;
-        ctl->launches = 0;
-        ctl->itmax = itmax;
-        ctl->e_last = 0.0;
-        ctl->tol = tol;
-        ctl->rtol = rtol;
-        ctl->n_last = 0.0;
+    if (blockIdx.x == 0 && threadIdx.x < 2) {  // both state slots; launch 0 reads slot 1
+        SorCtl* sl = ctl + threadIdx.x;
+        sl->k = 0;
+        sl->parity = 0;
+        sl->status = itmax > 0 ? SOR_RUNNING : SOR_FAILED;
+        sl->s_done = 0;
+        sl->launches = 0;
+        sl->itmax = itmax;
+        sl->e_last = 0.0;
+        sl->tol = tol;
+        sl->rtol = rtol;
+        sl->n_last = 0.0;
     }
 }
 
@@ -147,10 +148,10 @@ int dc_residual_launch(const double* w, const double* psi, float* r32, Diag*, in
     return launched();
 }
 
-int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCtl* ctl, int n, int ld,
+int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCtl* fin, int n, int ld,
                       hipStream_t st) {
     const long long count = (long long)n * ld;
-    hipLaunchKernelGGL(dc_correct_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, st, psi, e0, e1, ctl, count);
+    hipLaunchKernelGGL(dc_correct_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, st, psi, e0, e1, fin, count);
     return launched();
 }
 

@@ -98,17 +98,20 @@ __global__ void __launch_bounds__(kBlock) poisson_init_kernel(double* psi0, SorC
                                                              long long count, double tol, int itmax) {
     const long long stride = (long long)gridDim.x * kBlock;
     for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) psi0[q] = 0.0;
+    if (blockIdx.x == 0 && threadIdx.x < 2) {  // both state slots; launch 0 reads slot 1
+        SorCtl* sl = ctl + threadIdx.x;
+        sl->k = 0;
+        sl->parity = 0;
+        sl->status = itmax > 0 ? SOR_RUNNING : SOR_FAILED;
+        sl->s_done = 0;
+        sl->launches = 0;
+        sl->itmax = itmax;
+        sl->e_last = 0.0;
+        sl->tol = tol;
+        sl->rtol = 0.0;
+        sl->n_last = 0.0;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->k = 0;
-        ctl->parity = 0;
-        ctl->status = itmax > 0 ? SOR_RUNNING : SOR_FAILED;
-        ctl->s_rerun = 0;
-        ctl->launches = 0;
-        ctl->itmax = itmax;
-        ctl->e_last = 0.0;
-        ctl->tol = tol;
-        ctl->rtol = 0.0;
-        ctl->n_last = 0.0;
         if (diag) {
             diag->kmax = dkey(-DBL_MAX);
             diag->kmin = dkey(DBL_MAX);
@@ -119,13 +122,13 @@ __global__ void __launch_bounds__(kBlock) poisson_init_kernel(double* psi0, SorC
 
 // ---- velocities u = DY psi, v = -(DX psi) at every point (src/main.c:366-383) --------------
 __global__ void __launch_bounds__(kBlock) velocity_kernel(const double* psi0, const double* psi1,
-                                                         const SorCtl* ctl, double* __restrict__ u,
+                                                         const SorCtl* fin, double* __restrict__ u,
                                                          double* __restrict__ v, int n, int ld,
                                                          Stencils s) {
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= n || j >= n) return;
-    const double* psi = ctl->parity ? psi1 : psi0;
+    const double* psi = fin->parity ? psi1 : psi0;
     u[(size_t)i * ld + j] = sum_y(s.d1y, s.start, psi, i, j, ld);
     v[(size_t)i * ld + j] = -sum_x(s.d1x, s.start, psi, i, j, ld);
 }
@@ -163,81 +166,6 @@ __global__ void __launch_bounds__(kBlock) continuity_kernel(const double* __rest
     }
 }
 
-// ---- SOR decision: reduce the launch's per-tile partials in tile order, update the control
-//      block (convergence test of src/poisson.c:272-278 per sweep, rollback, budget, fold) ----
-__global__ void __launch_bounds__(kBlock) sor_decide_kernel(SorCtl* ctl, const double* __restrict__ partial,
-                                                           int ntiles, int S) {
-    constexpr int NW = kBlock / 64;
-    __shared__ double red[NW][kMaxSweepsPerLaunch][2];
-    const int status = ctl->status;
-    if (status != SOR_RUNNING && status != SOR_ROLLBACK) return;
-    const int k0 = ctl->k;
-    const int budget = ctl->itmax - k0;
-    const int s_eff = (status == SOR_ROLLBACK) ? ctl->s_rerun : (budget < S ? budget : S);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    // one pass: every thread loads all sweeps of its tiles at once, then one reduction
-    double e[kMaxSweepsPerLaunch], nn[kMaxSweepsPerLaunch];
-#pragma unroll
-    for (int s = 0; s < kMaxSweepsPerLaunch; ++s) e[s] = nn[s] = 0.0;
-    for (int t = tid; t < ntiles; t += kBlock) {
-#pragma unroll
-        for (int s = 0; s < kMaxSweepsPerLaunch; ++s) {
-            if (s < s_eff) {
-                e[s] += partial[((size_t)s * ntiles + t) * 2];
-                nn[s] += partial[((size_t)s * ntiles + t) * 2 + 1];
-            }
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < kMaxSweepsPerLaunch; ++s) {
-        const double a = wave_sum(e[s]), b = wave_sum(nn[s]);
-        if (lane == 0) {
-            red[wv][s][0] = a;
-            red[wv][s][1] = b;
-        }
-    }
-    __syncthreads();
-    if (tid != 0) return;
-    double etot[kMaxSweepsPerLaunch], ntot[kMaxSweepsPerLaunch];
-    for (int s = 0; s < s_eff; ++s) {
-        double a = 0.0, b = 0.0;
-        for (int w = 0; w < NW; ++w) {
-            a += red[w][s][0];
-            b += red[w][s][1];
-        }
-        etot[s] = a;
-        ntot[s] = b;
-    }
-    const double thr = ctl->tol, fold = ctl->rtol;
-    int s_conv = -1, s_fold = -1;
-    for (int s = 0; s < s_eff; ++s) {
-        if (etot[s] < thr) {
-            s_conv = s;
-            break;
-        }
-        if (fold > 0.0 && etot[s] < fold * ntot[s]) {
-            s_fold = s;
-            break;
-        }
-    }
-    if (s_conv >= 0 && s_conv < s_eff - 1) {
-        ctl->status = SOR_ROLLBACK;
-        ctl->s_rerun = s_conv + 1;
-    } else {
-        ctl->k = k0 + s_eff;
-        ctl->parity ^= 1;
-        ctl->e_last = etot[s_eff - 1];
-        ctl->n_last = ntot[s_eff - 1];
-        if (s_conv >= 0)
-            ctl->status = SOR_CONVERGED;
-        else if (s_fold >= 0)
-            ctl->status = SOR_FOLD;
-        else
-            ctl->status = (k0 + s_eff >= ctl->itmax) ? SOR_FAILED : SOR_RUNNING;
-    }
-    ctl->launches += 1;
-}
-
 // ---- legacy elementwise entry points -------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) euler_kernel(double* w, const double* dwdx, const double* dwdy,
                                                       const double* d2x, const double* d2y,
@@ -264,11 +192,6 @@ static inline unsigned blocks_for(long long count) { return (unsigned)((count + 
 
 int sor_fp64_launch(const SorArgs<double>& a, TileShape shape, bool gs, hipStream_t st) {
     return sor_dispatch<double>(a, shape, gs, st);
-}
-
-int sor_decide_launch(SorCtl* ctl, const double* partial, int ntiles, int S, hipStream_t st) {
-    hipLaunchKernelGGL(sor_decide_kernel, dim3(1), dim3(kBlock), 0, st, ctl, partial, ntiles, S);
-    return launched();
 }
 
 bool sor_shape_supported(TileShape sh) {
@@ -300,10 +223,10 @@ int poisson_init_launch(double* psi0, SorCtl* ctl, Diag* diag, int n, int ld, do
     return launched();
 }
 
-int velocity_launch(const double* psi0, const double* psi1, const SorCtl* ctl, double* u, double* v, int n,
+int velocity_launch(const double* psi0, const double* psi1, const SorCtl* fin, double* u, double* v, int n,
                     int ld, const Stencils& s, hipStream_t st) {
     dim3 grid((n + 63) / 64, (n + 3) / 4);
-    hipLaunchKernelGGL(velocity_kernel, grid, dim3(kBlock), 0, st, psi0, psi1, ctl, u, v, n, ld, s);
+    hipLaunchKernelGGL(velocity_kernel, grid, dim3(kBlock), 0, st, psi0, psi1, fin, u, v, n, ld, s);
     return launched();
 }
 

@@ -22,13 +22,14 @@
 // q = fma(fma(-q0, B, x), RN(1/B), q0): Markstein's correction step, which returns the
 // correctly rounded quotient RN(x/B) -- the same double as x/B -- for finite x.
 // The L1 norm of each sweep's update (src/poisson.c:34-60; sqrt(pow(d,2)) == |d|) is summed
-// over owned points per workgroup in a fixed order into partial[s][tile]; a one-workgroup
-// "decide" launch then reduces the partials in tile order (deterministic) and updates the
-// device control block (SorCtl):
-//   * a launch reads the status first and returns at once unless the solve is still running;
-//   * if sweep s < S-1 of a launch reaches e < tol, the launch's output buffer is discarded and
-//     the next launch redoes s+1 sweeps from the unchanged source buffer (ROLLBACK), so the
-//     final iterate is exactly the one of the first sweep with e < tol (src/poisson.c:273);
+// over owned points per workgroup in a fixed order into partial[L&1][s][tile].  The next launch
+// decides on them in its prologue, overlapped with its own (speculative) tile load: every
+// workgroup reduces the partials in tile order, so all reach the same decision, and workgroup 0
+// records it in the state slot L&1 (SorCtl) that the host polls:
+//   * a launch whose previous state is terminal copies it forward and returns;
+//   * if sweep s < S-1 of the previous launch reached e < tol, that launch's output is
+//     discarded: this launch redoes s+1 sweeps from its (untouched) input buffer, so the final
+//     iterate is exactly the one of the first sweep with e < tol (src/poisson.c:273);
 //   * the sweep budget itmax is honoured exactly (src/poisson.c:234);
 //   * defect correction (FP32 planes): the solve also stops ("fold") once the update norm fell
 //     below rtol times the iterate's own L1 norm, the resolution limit of FP32.
@@ -93,16 +94,22 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
 
     __shared__ T xch[2][NW][2][64];  // [half-sweep parity][wave][first/last row][lane]
     __shared__ double red[S][NW][2];
+    __shared__ double lred[S][2][NW * 64];  // previous launch's partials, one column per thread
+    __shared__ double etot[kMaxSweepsPerLaunch][2];
 
-    const SorCtl* ctl = a.ctl;
-    const int status = ctl->status;
-    if (status != SOR_RUNNING && status != SOR_ROLLBACK) return;
-    const int k0 = ctl->k;
-    const int budget = ctl->itmax - k0;
-    const int s_eff = (status == SOR_ROLLBACK) ? ctl->s_rerun : (budget < S ? budget : S);
-    const int parity = ctl->parity;
-    const T* __restrict__ src = parity ? a.psi1 : a.psi0;
-    T* __restrict__ dst = parity ? a.psi0 : a.psi1;
+    const int L = a.launch;
+    const SorCtl* prev = a.ctl + ((L + 1) & 1);
+    SorCtl* mine = a.ctl + (L & 1);
+    const int tid = threadIdx.x;
+    const int pstatus = prev->status;
+    if (pstatus != SOR_RUNNING) {  // the solve already ended: carry its state forward
+        if (blockIdx.x == 0 && tid == 0) *mine = *prev;
+        return;
+    }
+    const int sd = prev->s_done;   // sweeps the previous launch performed (0: first launch)
+    const int k_prev = prev->k;
+    const int itmax = prev->itmax;
+    const double tol = prev->tol, rtol = prev->rtol;
 
     const int ntiles = a.tiles_i * a.tiles_j;
     const int tile = sor_tile_of_block(blockIdx.x, ntiles);
@@ -115,6 +122,26 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (SGPR)
     const int row0 = wv * RB;        // even: the colour of (row0 + r, 2m + q) is (r + q) & 1
 
+    // the previous launch's per-tile partials: thread t sums tiles t, t + NT, ... in that order
+    const size_t pstride = (size_t)kMaxSweepsPerLaunch * ntiles * 2;
+    double pv[S][2];
+#pragma unroll
+    for (int s = 0; s < S; ++s) pv[s][0] = pv[s][1] = 0.0;
+    if (sd > 0) {
+        const double* pp = a.partial + ((L + 1) & 1) * pstride;
+        for (int t = tid; t < ntiles; t += NW * 64) {
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                if (s < sd) {
+                    pv[s][0] += pp[((size_t)s * ntiles + t) * 2];
+                    pv[s][1] += pp[((size_t)s * ntiles + t) * 2 + 1];
+                }
+            }
+        }
+    }
+
+    // speculative tile load for the common case (sweep on from the previous launch's output)
+    const T* __restrict__ srcN = (L & 1) ? a.psi1 : a.psi0;
     // column facts: q = 0 -> column 2m ("A"), q = 1 -> column 2m+1 ("B").  colh[q]: the point is
     // updated in half-sweep h iff colh[q] > h (distance to the extended tile's side, or -1 on
     // the domain boundary and outside the domain).
@@ -141,7 +168,7 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
         for (int q = 0; q < 2; ++q) {
             const int gj = gjq[q];
             T val = T(0), fv = T(0);
-            if (rin && (unsigned)gj < (unsigned)nc) val = src[(size_t)gi * ld + gj];
+            if (rin && (unsigned)gj < (unsigned)nc) val = srcN[(size_t)gi * ld + gj];
             if (rint && colh[q] >= 0) {
                 const T sv = a.src_f[(size_t)gi * ld + gj];
                 fv = a.negate ? -sv : sv;
@@ -150,6 +177,100 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
             fc[q][r] = a.c3 * fv;
         }
     }
+    // ---- decision on the previous launch: every workgroup reduces its per-tile partials in
+    //      the same fixed order and reaches the same decision (src/poisson.c:272-278) --------
+    if (sd > 0) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (s < sd) {
+                lred[s][0][tid] = pv[s][0];
+                lred[s][1][tid] = pv[s][1];
+            }
+        }
+        __syncthreads();
+        for (int row = wv; row < 2 * sd; row += NW) {  // wave w reduces (sweep, quantity) rows
+            const int s = row >> 1, q = row & 1;
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < NW; ++j) acc += lred[s][q][lane + 64 * j];
+            acc = wave_sum(acc);
+            if (lane == 0) etot[s][q] = acc;
+        }
+        __syncthreads();
+    }
+    int mode = 0;          // 0: sweep on, 1: stop (terminal), 2: redo s_run sweeps (rollback)
+    int s_run = 0;
+    SorCtl next;
+    next.k = k_prev;
+    next.parity = L & 1;   // where the previous launch left its output
+    next.status = SOR_RUNNING;
+    next.itmax = itmax;
+    next.tol = tol;
+    next.rtol = rtol;
+    next.launches = prev->launches;
+    next.e_last = prev->e_last;
+    next.n_last = prev->n_last;
+    if (sd > 0) {
+        int s_conv = -1, s_fold = -1;
+        for (int s = 0; s < sd; ++s) {
+            if (etot[s][0] < tol) {
+                s_conv = s;
+                break;
+            }
+            if (rtol > 0.0 && etot[s][0] < rtol * etot[s][1]) {
+                s_fold = s;
+                break;
+            }
+        }
+        if (s_conv >= 0 && s_conv < sd - 1) {
+            // converged inside the previous launch: redo s_conv+1 sweeps from its (untouched)
+            // input buffer; that iterate is the final one (src/poisson.c:273)
+            mode = 2;
+            s_run = s_conv + 1;
+            next.status = SOR_CONVERGED;
+            next.k = k_prev + s_run;
+            next.e_last = etot[s_conv][0];
+            next.n_last = etot[s_conv][1];
+        } else {
+            next.k = k_prev + sd;
+            next.e_last = etot[sd - 1][0];
+            next.n_last = etot[sd - 1][1];
+            if (s_conv >= 0) {
+                next.status = SOR_CONVERGED;
+                mode = 1;
+            } else if (s_fold >= 0) {
+                next.status = SOR_FOLD;
+                mode = 1;
+            } else if (next.k >= itmax) {
+                next.status = SOR_FAILED;
+                mode = 1;
+            }
+        }
+    } else if (itmax <= k_prev) {
+        next.status = SOR_FAILED;
+        mode = 1;
+    }
+    if (mode == 0) s_run = (itmax - next.k) < S ? (itmax - next.k) : S;
+    next.s_done = (mode == 0) ? s_run : 0;
+    if (mode != 1) next.launches += 1;
+    if (blockIdx.x == 0 && tid == 0) *mine = next;
+    if (mode == 1) return;
+    const int s_eff = s_run;
+    // normal: read the previous launch's output, write the other buffer; rollback: read the
+    // previous launch's input, write over its output
+    T* __restrict__ dst = ((L & 1) ^ (mode == 2 ? 0 : 1)) ? a.psi1 : a.psi0;
+    if (mode == 2) {
+        const T* __restrict__ srcR = (L & 1) ? a.psi0 : a.psi1;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int gi = gi0 + row0 + r;
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+                v[q][r] = ((unsigned)gi < (unsigned)nr && (unsigned)gjq[q] < (unsigned)nc)
+                              ? srcR[(size_t)gi * ld + gjq[q]] : T(0);
+        }
+    }
+
     // half-sweep 0 updates colour 0 and reads its neighbours' colour-1 boundary rows:
     // colour 1 sits in column q = (r + 1) & 1 of row r
     xch[1][wv][0][lane] = v[1][0];
@@ -216,13 +337,12 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
         }
     }
     __syncthreads();
-    const int tid = threadIdx.x;
-    if (tid < 2 * s_eff) {
+    if (mode == 0 && tid < 2 * s_eff) {  // a rollback launch ends the solve: no partials needed
         const int s = tid >> 1, q = tid & 1;
         double t = 0.0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) t += red[s][w][q];
-        a.partial[((size_t)s * ntiles + tile) * 2 + q] = t;
+        a.partial[(L & 1) * pstride + ((size_t)s * ntiles + tile) * 2 + q] = t;
     }
 }
 
