@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-dc", action="store_true", help="skip the FP32-refined leg")
     ap.add_argument("--sor-launches", type=int, default=400)
+    ap.add_argument("--profile-sor", type=int, default=0, metavar="LAUNCHES",
+                    help="profiling mode: one step, then LAUNCHES timed SOR launches and one 256 MiB "
+                         "calibration copy (for counter scaling); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the single cavity would "
@@ -115,6 +118,16 @@ def main():
     dev = cnv.device_name(0)
     print("bench: %s, %dx%d Re=%g, %d warmup + %d timed steps" % (dev, N, N, RE, args.warmup, args.steps),
           file=sys.stderr, flush=True)
+
+    if args.profile_sor:
+        cav = cnv.Cavity(workload_params(cnv, cnv.PREC_FP64, args))
+        timed_steps(cav, 1, "fp64")
+        ms = cav.time_sor(args.profile_sor)
+        cms = cav.calibration_copy(256 << 20)
+        print("bench: profile mode: SOR launch %.4f ms, calibration copy of 256 MiB %.4f ms" % (ms, cms),
+              file=sys.stderr, flush=True)
+        cav.close()
+        return
 
     # ---- FP64 plain ---------------------------------------------------------------------
     cav = cnv.Cavity(workload_params(cnv, cnv.PREC_FP64, args))

@@ -72,6 +72,7 @@ def lib():
         L.cnv_sub_host.argtypes = [C.c_int, C.c_int, _DP, _DP, _DP]
         L.cnv_time_sor.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.cnv_sweeps_per_launch.argtypes = [C.c_void_p]
+        L.cnv_calibration_copy.argtypes = [C.c_void_p, C.c_longlong, C.POINTER(C.c_double)]
         L.cnv_device_name.argtypes = [C.c_int, C.c_char_p, C.c_int]
         _lib = L
     return _lib
@@ -176,6 +177,11 @@ class Cavity:
     def time_sor(self, launches):
         ms = C.c_double()
         _check(lib().cnv_time_sor(self.h, launches, C.byref(ms)), self.h)
+        return ms.value
+
+    def calibration_copy(self, nbytes):
+        ms = C.c_double()
+        _check(lib().cnv_calibration_copy(self.h, nbytes, C.byref(ms)), self.h)
         return ms.value
 
     @property

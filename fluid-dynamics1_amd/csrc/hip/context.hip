@@ -643,3 +643,21 @@ int cnv_sub_host(int m, int n, const double* a, const double* b, double* out) {
     if (m <= 0 || n <= 0) return fail(nullptr, CNV_ERR_ARG, "bad size");
     return elementwise_host(m, n, a, b, out, 1);
 }
+
+int cnv_calibration_copy(cnv_ctx* c, long long nbytes, double* ms) {
+    if (!c || nbytes <= 0 || nbytes % 8) return fail(c, CNV_ERR_ARG, "calibration copy needs a positive multiple of 8 bytes");
+    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    double *a = nullptr, *b = nullptr;
+    HIPC(c, hipMalloc(&a, nbytes));
+    HIPC(c, hipMalloc(&b, nbytes));
+    int rc = CNV_OK;
+    if (copy_launch(b, a, nbytes / 8, c->stream)) rc = fail(c, CNV_ERR_HIP, "copy kernel failed");  // warm
+    if (rc == CNV_OK && (hipEventRecord(c->t0, c->stream) != hipSuccess || copy_launch(b, a, nbytes / 8, c->stream) ||
+                         hipEventRecord(c->t1, c->stream) != hipSuccess || hipEventSynchronize(c->t1) != hipSuccess))
+        rc = fail(c, CNV_ERR_HIP, "copy kernel failed");
+    float t = 0.f;
+    if (rc == CNV_OK && hipEventElapsedTime(&t, c->t0, c->t1) == hipSuccess && ms) *ms = t;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}

@@ -113,6 +113,9 @@ int cnv_sub_host(int m, int n, const double *a, const double *b, double *out);
  * HIP events on the stream the kernel runs on.  Restores nothing: call on a scratch context. */
 int cnv_time_sor(cnv_ctx *ctx, int launches, double *mean_ms);
 int cnv_sweeps_per_launch(const cnv_ctx *ctx);
+/* Counter calibration: copies nbytes between two scratch device buffers with a plain copy kernel
+ * (nbytes read + nbytes written), so that profiler byte counters can be scaled. */
+int cnv_calibration_copy(cnv_ctx *ctx, long long nbytes, double *ms);
 int cnv_device_name(int device, char *buf, int len);
 
 #ifdef __cplusplus
