@@ -661,3 +661,17 @@ int cnv_calibration_copy(cnv_ctx* c, long long nbytes, double* ms) {
     (void)hipFree(b);
     return rc;
 }
+
+int cnv_set_poisson_params(cnv_ctx* c, double dx, double dy, double beta, double tol, int itmax, int poisson_type) {
+    if (!c) return fail(nullptr, CNV_ERR_ARG, "null context");
+    if (poisson_type != 1 && poisson_type != 2)
+        return fail(c, CNV_ERR_ARG, "Error - invalid option for Poisson solver (poisson_type=%d)", poisson_type);
+    if (!(dx > 0) || !(dy > 0)) return fail(c, CNV_ERR_ARG, "grid spacing must be positive");
+    c->p.dx = dx;
+    c->p.dy = dy;
+    c->p.beta = beta;
+    c->p.tol = tol;
+    c->p.itmax = itmax;
+    c->p.poisson_type = poisson_type;
+    return CNV_OK;
+}

@@ -91,6 +91,12 @@ int cnv_step_timed(cnv_ctx *ctx, int nsteps, cnv_step_stats *stats, double *elap
 /* Device pointers of the state (w, psi, u, v) for zero-copy callers; valid until the next step. */
 int cnv_device_state(cnv_ctx *ctx, double **w, double **psi, double **u, double **v);
 
+/* Changes the Poisson solve's parameters of an existing context (spacing used by the SOR
+ * update, relaxation, tolerance, sweep budget, 1 = Gauss-Seidel / 2 = SOR).  The transport
+ * stencils keep the spacing the context was created with. */
+int cnv_set_poisson_params(cnv_ctx *ctx, double dx, double dy, double beta, double tol, int itmax,
+                           int poisson_type);
+
 /* Poisson solve alone on this context's grid: psi = SOR(f) from zero (src/poisson.c:224-285),
  * f given as a host array; psi written to host.  k/e as logged by the reference. */
 int cnv_poisson_host(cnv_ctx *ctx, const double *f, double *psi, int *k, double *e);

@@ -70,6 +70,46 @@ def main():
     np.savez_compressed(os.path.join(HERE, "elementwise.npz"), Re=737.0, dt=0.0031, euler_out=out,
                         continuity_out=cont, vorticity_out=vort, **{"in_" + k: v for k, v in ins.items()})
 
+    # --- host helpers: linearalg, config parser, VTK writer ---------------------------------
+    host = {}
+    A = rng.standard_normal((6, 5))
+    B = rng.standard_normal((5, 3))
+    Am, Bm = ser.to_m(A), ser.to_m(B)
+    Cm = ser.L.mtrxmul(Am, Bm)
+    host["mul_A"], host["mul_B"], host["mul_C"] = A, B, ser.from_m(Cm)
+    K1, K2 = rng.standard_normal((3, 3)), rng.standard_normal((3, 3))
+    k1, k2 = ser.to_m(K1), ser.to_m(K2)
+    Km = ser.L.kronecker(k1, k2)
+    host["kron_A"], host["kron_B"], host["kron_C"] = K1, K2, ser.from_m(Km)
+    Rm = ser.L.reshape(Am, 3, 10)
+    host["reshape_in"], host["reshape_out"] = A, ser.from_m(Rm)
+    host["maxel"] = np.array(ser.L.maxel(Am))
+    host["minel"] = np.array(ser.L.minel(Am))
+    ser.free(Am, Bm, Cm, k1, k2, Km, Rm)
+    np.savez_compressed(os.path.join(HERE, "host_linearalg.npz"), **host)
+
+    cfg_text = ("# comment line\n; another comment\n\nRe = 2500.5\nnx=48\n  ny   =   48   \n"
+                "dt = 0.002 # trailing comment is dropped\ntf = 1.5\norder = 4\npoisson_tol = 2.5E-7\n"
+                "poisson_type = 1\nbogus_key = 3\nu4 = 1.25\nv2=-0.5\nLx = 2\noutput_interval = 7\n"
+                "poisson_max_it = 1234\nmax_co = 0.75\nopenmp_enabled = 1\nui = 0.125\n")
+    cfg_path = os.path.join(HERE, "config_parser_input.txt")
+    with open(cfg_path, "w") as fh:
+        fh.write(cfg_text)
+    cfg = ser.L.load_config_from_file(cfg_path.encode())
+    np.savez_compressed(os.path.join(HERE, "config_parser.npz"),
+                        **{name: np.array(getattr(cfg, name)) for name, _ in refbind.Config._fields_})
+
+    import tempfile
+    tmp = tempfile.mkdtemp()
+    V = rng.standard_normal((5, 7)) * 3
+    Vm = ser.to_m(V)
+    ser.L.printvtk.argtypes = [refbind.Mtrx, refbind.C.c_char_p, refbind.C.c_char_p]
+    ser.L.printvtk(Vm, b"field", tmp.encode())
+    with open(os.path.join(tmp, "field-1-0.vtk")) as fh:
+        vtk_text = fh.read()
+    ser.free(Vm)
+    np.savez_compressed(os.path.join(HERE, "vtk_writer.npz"), values=V, text=np.array(vtk_text))
+
     # --- Poisson --------------------------------------------------------------------------
     pois = {}
     for idx, (n, beta, use_omp, tol) in enumerate(POISSON_CASES):

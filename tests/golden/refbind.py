@@ -5,7 +5,7 @@ only calls the functions it exports (include/linearalg.h, finitediff.h, poisson.
 config.h of the reference).  Used by make_golden.py to write fixtures and by the CPU tests that
 cross-check the oracle against the reference when oracle/_ref is present.
 """
-import ctypes as C
+import ctypes as C  # noqa: N812
 import os
 import re
 import tempfile
