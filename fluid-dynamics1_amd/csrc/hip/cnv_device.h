@@ -81,6 +81,8 @@ struct SorArgs {
     int negate;          // 1: f = -src (the reference's invsig before the solve)
     T c1, c2, c3, B, beta, omb;  // dy*dy, dx*dx, dx*dx*dy*dy, 2*(dx*dx+dy*dy), beta, 1-beta
     T rB;                // RN(1/B)
+    int exact_rcp;       // 1: 1/B is exact (B a power of two), so x*rB == RN(x/B)
+    int vec2;            // 1: ld is even and the buffers are 2*sizeof(T)-aligned (paired access)
 };
 
 // ---- host launch wrappers (defined in the .hip files) ---------------------------------

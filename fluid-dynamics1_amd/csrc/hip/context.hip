@@ -17,6 +17,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <cmath>
 #include <vector>
 
 #include "cnv.h"
@@ -344,6 +345,14 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
 int cnv_sweeps_per_launch(const cnv_ctx* c) { return c->shape.s; }
 
 // ---- SOR driver ---------------------------------------------------------------------------------
+// Paired (2-element vector) access needs an even row length and pitch and 2*sizeof(T)-aligned
+// buffers.
+template <typename T>
+static int sor_vec2(const SorArgs<T>& a) {
+    const uintptr_t m = 2 * sizeof(T) - 1;
+    return (a.nc % 2 == 0) && (a.ld % 2 == 0) && !((uintptr_t)a.src_f & m) && !((uintptr_t)a.psi0 & m) && !((uintptr_t)a.psi1 & m);
+}
+
 template <typename T>
 static SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int negate) {
     SorArgs<T> a{};
@@ -367,6 +376,10 @@ static SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int negate) {
     a.beta = (T)c->p.beta;
     a.omb = (T)(1 - c->p.beta);
     a.rB = (T)(1.0 / (2 * (dx * dx + dy * dy)));
+    int e2 = 0;
+    const double Bd = 2 * (dx * dx + dy * dy);
+    a.exact_rcp = std::frexp(Bd, &e2) == 0.5 && std::isnormal(1.0 / Bd) && (1.0 / Bd) * Bd == 1.0;
+    a.vec2 = sor_vec2(a);
     return a;
 }
 

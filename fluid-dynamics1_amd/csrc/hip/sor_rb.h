@@ -13,14 +13,20 @@
 //     registers (rows r+-1), one of its E/W neighbours is the lane's other column and the other
 //     comes from lane m+-1 through a DPP wave shift; only the first and last row of each wave
 //     are exchanged through LDS, once per half-sweep;
-//   * every point of a row is computed and those outside the valid rectangle or on the domain
-//     boundary keep their value (select), so the sweep is straight-line code.
+//   * every point of a row is computed, so the sweep is straight-line code with no selects.  A
+//     point outside the valid rectangle takes a wrong (finite) value, which only points outside
+//     the valid rectangle ever read.  Domain-boundary points (and points outside the domain) must
+//     keep psi = 0: their relaxation factor is 0 instead of beta, which leaves them at +-0, and
+//     the final store writes +0 there;
+//   * the L1 norms are accumulated per lane with a 0/1 weight per row (owned rows only) and a
+//     per-column ownership mask, written to LDS once per sweep and reduced once per launch.
 // Point update (src/poisson.c:268 SOR, src/poisson.c:205 Gauss-Seidel), same association:
 //   u = beta*(dy*dy*(N+S) + dx*dx*(E+W) - dx*dx*dy*dy*f) / (2*(dx*dx+dy*dy)) + (1-beta)*u0
 // where u0 (the value before this sweep) is the point's current value in either colouring.
 // In FP64 the division by the constant B = 2*(dx*dx+dy*dy) is q0 = x*RN(1/B),
 // q = fma(fma(-q0, B, x), RN(1/B), q0): Markstein's correction step, which returns the
-// correctly rounded quotient RN(x/B) -- the same double as x/B -- for finite x.
+// correctly rounded quotient RN(x/B) -- the same double as x/B -- for finite x.  When B is a
+// power of two (dx = dy = 2^-k), 1/B is exact and x*(1/B) is already RN(x/B): one multiply (XR).
 // The L1 norm of each sweep's update (src/poisson.c:34-60; sqrt(pow(d,2)) == |d|) is summed
 // over owned points per workgroup in a fixed order into partial[L&1][s][tile].  The next launch
 // decides on them in its prologue, overlapped with its own (speculative) tile load: every
@@ -67,21 +73,59 @@ __device__ inline double sor_div(double x, double B, double rB) {
 // FP32 planes (defect correction): no bit-parity requirement, one multiply by RN(1/B)
 __device__ inline float sor_div(float x, float, float rB) { return x * rB; }
 
-// Lane i receives lane i+1's value (wave_shl:1) / lane i-1's value (wave_shr:1).
+// The lane's column pair (gj, gj+1) of one row, gj even.  vec (wave-uniform, checked by the
+// host: nc and ld even, buffers 2*sizeof(T)-aligned): one 2-element access whenever the pair lies
+// inside the row (pair), with the columns outside in0 / in1 read as 0 or left unwritten.
+// Otherwise per column, dereferencing only columns inside in0 / in1.
+template <typename T>
+__device__ inline void load_pair(const T* __restrict__ p, long long off, bool pair, bool in0, bool in1,
+                                 bool vec, T& x0, T& x1) {
+    typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type V2;
+    if (vec) {
+        V2 d;
+        d.x = T(0);
+        d.y = T(0);
+        if (pair) d = *reinterpret_cast<const V2*>(p + off);
+        x0 = in0 ? d.x : T(0);
+        x1 = in1 ? d.y : T(0);
+    } else {
+        x0 = in0 ? p[off] : T(0);
+        x1 = in1 ? p[off + 1] : T(0);
+    }
+}
+template <typename T>
+__device__ inline void store_pair(T* __restrict__ p, long long off, bool in0, bool in1, bool vec, T x0,
+                                  T x1) {
+    typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type V2;
+    if (vec) {  // nc even: in0 == in1 for an even gj
+        if (in0) {
+            V2 d;
+            d.x = x0;
+            d.y = x1;
+            *reinterpret_cast<V2*>(p + off) = d;
+        }
+    } else {
+        if (in0) p[off] = x0;
+        if (in1) p[off + 1] = x1;
+    }
+}
+
+// Lane i receives lane i+1's value (wave_shl:1) / lane i-1's value (wave_shr:1); the lane
+// without a source (63 / 0) reads 0 (bound_ctrl), so no old value has to be materialised.
 template <int CTRL>
 __device__ inline double dpp_shift(double x) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, true);
     return __hiloint2double(hi, lo);
 }
 template <int CTRL>
 __device__ inline float dpp_shift(float x) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, true));
 }
 constexpr int kDppFromNext = 0x130;  // wave_shl:1
 constexpr int kDppFromPrev = 0x138;  // wave_shr:1
 
-template <typename T, int S, int NW, int RB, bool GS>
+template <typename T, int S, int NW, int RB, bool GS, bool XR>
 __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
     constexpr int H = 2 * S;
     constexpr int EJ = 128;          // extended tile columns; lane m holds columns 2m, 2m+1
@@ -93,14 +137,73 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
     typedef typename std::conditional<sizeof(T) == 4, float, double>::type Acc;
 
     __shared__ T xch[2][NW][2][64];  // [half-sweep parity][wave][first/last row][lane]
-    __shared__ double red[S][NW][2];
-    __shared__ double lred[S][2][NW * 64];  // previous launch's partials, one column per thread
+    // one column per thread: the previous launch's partials (prologue), then this launch's
+    // per-lane L1 sums (sweep loop)
+    __shared__ double lred[S][2][NW * 64];
     __shared__ double etot[kMaxSweepsPerLaunch][2];
 
     const int L = a.launch;
     const SorCtl* prev = a.ctl + ((L + 1) & 1);
     SorCtl* mine = a.ctl + (L & 1);
     const int tid = threadIdx.x;
+    const int ntiles = a.tiles_i * a.tiles_j;
+    const int tile = sor_tile_of_block(blockIdx.x, ntiles);
+    const int ti = tile / a.tiles_j;
+    const int tj = tile - ti * a.tiles_j;
+    const int gi0 = ti * TI - H;  // even, so the colour of local (li, lj) is (li + lj) & 1
+    const int gj0 = tj * TJ - H;
+    const int nr = a.nr, nc = a.nc, ld = a.ld;
+    const bool vec = a.vec2 != 0;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (SGPR)
+    const int row0 = wv * RB;        // even: the colour of (row0 + r, 2m + q) is (r + q) & 1
+
+    // Speculative tile load for the common case (sweep on from the previous launch's output),
+    // issued before the state slot and the partials are read so that all three are in flight
+    // together.  Column facts: q = 0 -> column 2m, q = 1 -> column 2m+1.
+    const T* __restrict__ srcN = (L & 1) ? a.psi1 : a.psi0;
+    bool colin[2], ownc[2], colv[2];
+    int gjq[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int lj = 2 * lane + q;
+        gjq[q] = gj0 + lj;
+        colv[q] = (unsigned)gjq[q] < (unsigned)nc;
+        colin[q] = gjq[q] >= 1 && gjq[q] <= nc - 2;
+        ownc[q] = lj >= H && lj < H + TJ && gjq[q] < nc;
+    }
+    // relaxation factor of an updated point: FP64 beta (1 for Gauss-Seidel), applied before the
+    // division; FP32 beta/B (1/B), which folds the division in
+    T bcoef;
+    if constexpr (sizeof(T) == 8) bcoef = GS ? T(1) : a.beta;
+    else bcoef = GS ? a.rB : a.beta * a.rB;
+
+    T v[2][RB];   // v[q][r]: value at (row0 + r, 2*lane + q)
+    T fc[2][RB];  // FP64: c3*f (the hoisted c3*f product of the update); FP32: -c3*f
+    T be[2][RB];  // bcoef at interior points, 0 on the domain boundary and outside the domain
+    Acc w[RB];    // 1 for owned rows inside the domain, else 0 (wave-uniform)
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        const int li = row0 + r;
+        const int gi = gi0 + li;
+        const bool rin = (unsigned)gi < (unsigned)nr;
+        const bool rint = gi >= 1 && gi <= nr - 2;
+        w[r] = (li >= H && li < H + TI && gi < nr) ? Acc(1) : Acc(0);
+        const long long off = (long long)(rin ? gi : 0) * ld + gjq[0];
+        T f0, f1;
+        load_pair(srcN, off, rin && colv[0], rin && colv[0], rin && colv[1], vec, v[0][r], v[1][r]);
+        load_pair(a.src_f, off, rint && colv[0], rint && colin[0], rint && colin[1], vec, f0, f1);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const bool inter = rint && colin[q];
+            const T fq = q ? f1 : f0;
+            const T fv = a.negate ? -fq : fq;
+            if constexpr (sizeof(T) == 8) fc[q][r] = a.c3 * fv;
+            else fc[q][r] = -(a.c3 * fv);
+            be[q][r] = inter ? bcoef : T(0);
+        }
+    }
+
     const int pstatus = prev->status;
     if (pstatus != SOR_RUNNING) {  // the solve already ended: carry its state forward
         if (blockIdx.x == 0 && tid == 0) *mine = *prev;
@@ -111,70 +214,20 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
     const int itmax = prev->itmax;
     const double tol = prev->tol, rtol = prev->rtol;
 
-    const int ntiles = a.tiles_i * a.tiles_j;
-    const int tile = sor_tile_of_block(blockIdx.x, ntiles);
-    const int ti = tile / a.tiles_j;
-    const int tj = tile - ti * a.tiles_j;
-    const int gi0 = ti * TI - H;  // even, so the colour of local (li, lj) is (li + lj) & 1
-    const int gj0 = tj * TJ - H;
-    const int nr = a.nr, nc = a.nc, ld = a.ld;
-    const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (SGPR)
-    const int row0 = wv * RB;        // even: the colour of (row0 + r, 2m + q) is (r + q) & 1
-
     // the previous launch's per-tile partials: thread t sums tiles t, t + NT, ... in that order
+    // (all S slots are read; only the first sd are used)
     const size_t pstride = (size_t)kMaxSweepsPerLaunch * ntiles * 2;
     double pv[S][2];
 #pragma unroll
     for (int s = 0; s < S; ++s) pv[s][0] = pv[s][1] = 0.0;
-    if (sd > 0) {
+    {
         const double* pp = a.partial + ((L + 1) & 1) * pstride;
         for (int t = tid; t < ntiles; t += NW * 64) {
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                if (s < sd) {
-                    pv[s][0] += pp[((size_t)s * ntiles + t) * 2];
-                    pv[s][1] += pp[((size_t)s * ntiles + t) * 2 + 1];
-                }
+                pv[s][0] += pp[((size_t)s * ntiles + t) * 2];
+                if (sizeof(T) == 4) pv[s][1] += pp[((size_t)s * ntiles + t) * 2 + 1];
             }
-        }
-    }
-
-    // speculative tile load for the common case (sweep on from the previous launch's output)
-    const T* __restrict__ srcN = (L & 1) ? a.psi1 : a.psi0;
-    // column facts: q = 0 -> column 2m ("A"), q = 1 -> column 2m+1 ("B").  colh[q]: the point is
-    // updated in half-sweep h iff colh[q] > h (distance to the extended tile's side, or -1 on
-    // the domain boundary and outside the domain).
-    int colh[2];
-    bool ownc[2];
-    int gjq[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int lj = 2 * lane + q;
-        gjq[q] = gj0 + lj;
-        const int dj = lj < EJ - 1 - lj ? lj : EJ - 1 - lj;
-        colh[q] = (gjq[q] >= 1 && gjq[q] <= nc - 2) ? dj : -1;
-        ownc[q] = lj >= H && lj < H + TJ && gjq[q] < nc;
-    }
-
-    T v[2][RB];   // v[q][r]: value at (row0 + r, 2*lane + q)
-    T fc[2][RB];  // fc[q][r]: c3 * f at the same point (the hoisted c3*f product of the update)
-#pragma unroll
-    for (int r = 0; r < RB; ++r) {
-        const int gi = gi0 + row0 + r;
-        const bool rin = (unsigned)gi < (unsigned)nr;
-        const bool rint = gi >= 1 && gi <= nr - 2;
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int gj = gjq[q];
-            T val = T(0), fv = T(0);
-            if (rin && (unsigned)gj < (unsigned)nc) val = srcN[(size_t)gi * ld + gj];
-            if (rint && colh[q] >= 0) {
-                const T sv = a.src_f[(size_t)gi * ld + gj];
-                fv = a.negate ? -sv : sv;
-            }
-            v[q][r] = val;
-            fc[q][r] = a.c3 * fv;
         }
     }
     // ---- decision on the previous launch: every workgroup reduces its per-tile partials in
@@ -188,8 +241,11 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
             }
         }
         __syncthreads();
-        for (int row = wv; row < 2 * sd; row += NW) {  // wave w reduces (sweep, quantity) rows
-            const int s = row >> 1, q = row & 1;
+        // wave w reduces (sweep, quantity) rows; FP64 solves carry no iterate norm
+        const int nq = sizeof(T) == 4 ? 2 : 1;
+        if (nq == 1 && tid < kMaxSweepsPerLaunch) etot[tid][1] = 0.0;
+        for (int row = wv; row < nq * sd; row += NW) {
+            const int s = row / nq, q = row - s * nq;
             double acc = 0.0;
 #pragma unroll
             for (int j = 0; j < NW; ++j) acc += lred[s][q][lane + 64 * j];
@@ -264,10 +320,9 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
             const int gi = gi0 + row0 + r;
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-                v[q][r] = ((unsigned)gi < (unsigned)nr && (unsigned)gjq[q] < (unsigned)nc)
-                              ? srcR[(size_t)gi * ld + gjq[q]] : T(0);
+            const bool rin = (unsigned)gi < (unsigned)nr;
+            load_pair(srcR, (long long)(rin ? gi : 0) * ld + gjq[0], rin && colv[0], rin && colv[0],
+                      rin && colv[1], vec, v[0][r], v[1][r]);
         }
     }
 
@@ -275,84 +330,91 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
     // colour 1 sits in column q = (r + 1) & 1 of row r
     xch[1][wv][0][lane] = v[1][0];
     xch[1][wv][1][lane] = v[(RB - 1 + 1) & 1][RB - 1];
-    __syncthreads();
+    __syncthreads();  // also: every wave is done reading lred in the prologue
 
-    const T c1 = a.c1, c2 = a.c2, B = a.B, rB = a.rB, beta = a.beta, omb = a.omb;
+    const T c1 = a.c1, c2 = a.c2, B = a.B, rB = a.rB, omb = a.omb;
     for (int s = 0; s < s_eff; ++s) {
         Acc acc[2] = {Acc(0), Acc(0)}, nrm[2] = {Acc(0), Acc(0)};
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const int h = 2 * s + c;
             // boundary rows of the neighbouring waves (colour 1-c, written by half-sweep h-1)
             const T sb = xch[c ^ 1][wv > 0 ? wv - 1 : 0][1][lane];
             const T nb = xch[c ^ 1][wv < NW - 1 ? wv + 1 : NW - 1][0][lane];
-            T nv[RB];
+            // rows 1..RB-2 first: they do not wait for the LDS reads of sb / nb
 #pragma unroll
-            for (int r = 0; r < RB; ++r) {
+            for (int rr = 0; rr < RB; ++rr) {
+                const int r = rr < RB - 2 ? rr + 1 : (rr == RB - 2 ? 0 : RB - 1);
                 const int q = (c + r) & 1;  // column of colour c in row r (compile-time)
-                const int li = row0 + r;    // wave-uniform from here on
-                const int gi = gi0 + li;
-                const int rowd = li < EI - 1 - li ? li : EI - 1 - li;
-                const bool rowok = rowd > h && gi >= 1 && gi <= nr - 2;
                 const T old = v[q][r];
                 const T up = (r < RB - 1) ? v[q][r + 1] : nb;
                 const T dn = (r > 0) ? v[q][r - 1] : sb;
                 // E + W: the lane's other column plus the neighbouring lane's nearer column
                 const T own = v[q ^ 1][r];
                 const T far = q ? dpp_shift<kDppFromNext>(v[0][r]) : dpp_shift<kDppFromPrev>(v[1][r]);
-                const T rt = q ? far : own;
-                const T lt = q ? own : far;
-                const T x = c1 * (up + dn) + c2 * (rt + lt) - fc[q][r];
-                const T cand = GS ? sor_div(x, B, rB) : sor_div(beta * x, B, rB) + omb * old;
-                nv[r] = (rowok && colh[q] > h) ? cand : old;
-                if (li >= H && li < H + TI) {  // owned row (uniform branch); columns masked below
-                    acc[q] += (Acc)fabs(nv[r] - old);
-                    if (sizeof(T) == 4) nrm[q] += (Acc)fabs(nv[r]);
+                T cand;
+                if constexpr (sizeof(T) == 8) {
+                    // the reference's association: ((c1*(N+S) + c2*(E+W)) - c3*f), times beta,
+                    // divided by B, plus (1-beta)*u0; + is commutative, so E/W order is free
+                    const T x = c1 * (up + dn) + c2 * (own + far) - fc[q][r];
+                    const T y = be[q][r] * x;
+                    const T qt = XR ? y * rB : sor_div(y, B, rB);
+                    cand = GS ? qt : qt + omb * old;
+                    v[q][r] = cand;
+                    acc[q] = __fma_rn(w[r], fabs(cand - old), acc[q]);
+                } else {
+                    const T x = __fmaf_rn(c1, up + dn, __fmaf_rn(c2, own + far, fc[q][r]));
+                    const T y = be[q][r] * x;
+                    cand = GS ? y : __fmaf_rn(omb, old, y);
+                    v[q][r] = cand;
+                    acc[q] = __fmaf_rn(w[r], fabsf(cand - old), acc[q]);
+                    nrm[q] = __fmaf_rn(w[r], fabsf(cand), nrm[q]);
                 }
             }
-#pragma unroll
-            for (int r = 0; r < RB; ++r) v[(c + r) & 1][r] = nv[r];
             xch[c][wv][0][lane] = v[c & 1][0];
             xch[c][wv][1][lane] = v[(c + RB - 1) & 1][RB - 1];
             __syncthreads();
         }
-        const Acc eacc = (ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0));
-        const Acc nacc = (ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0));
-        const double we = wave_sum((double)eacc);
-        const double wn = sizeof(T) == 4 ? wave_sum((double)nacc) : 0.0;
-        if (lane == 0) {
-            red[s][wv][0] = we;
-            red[s][wv][1] = wn;
-        }
+        // per-lane sums of this sweep, owned columns only; reduced once after the last sweep
+        lred[s][0][tid] = (double)((ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0)));
+        if (sizeof(T) == 4)
+            lred[s][1][tid] = (double)((ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0)));
     }
 
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
         const int li = row0 + r;
         const int gi = gi0 + li;
-        if (li >= H && li < H + TI && gi < nr) {
-#pragma unroll
-            for (int q = 0; q < 2; ++q)
-                if (ownc[q]) dst[(size_t)gi * ld + gjq[q]] = v[q][r];
+        if (li >= H && li < H + TI && gi < nr) {  // owned row (wave-uniform)
+            const bool rint = gi >= 1 && gi <= nr - 2;
+            // domain-boundary points hold +-0: store +0
+            store_pair(dst, (long long)gi * ld + gjq[0], ownc[0], ownc[1], vec,
+                       (rint && colin[0]) ? v[0][r] : T(0), (rint && colin[1]) ? v[1][r] : T(0));
         }
     }
+    if (mode != 0) return;  // a rollback launch ends the solve: no partials needed
     __syncthreads();
-    if (mode == 0 && tid < 2 * s_eff) {  // a rollback launch ends the solve: no partials needed
-        const int s = tid >> 1, q = tid & 1;
-        double t = 0.0;
+    // this launch's per-tile partials: wave w reduces (sweep, quantity) rows, in a fixed order
+    const int nq = sizeof(T) == 4 ? 2 : 1;
+    for (int row = wv; row < nq * s_eff; row += NW) {
+        const int s = row / nq, q = row - s * nq;
+        double acc = 0.0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) t += red[s][w][q];
-        a.partial[(L & 1) * pstride + ((size_t)s * ntiles + tile) * 2 + q] = t;
+        for (int j = 0; j < NW; ++j) acc += lred[s][q][lane + 64 * j];
+        acc = wave_sum(acc);
+        if (lane == 0) a.partial[(L & 1) * pstride + ((size_t)s * ntiles + tile) * 2 + q] = acc;
     }
 }
 
 template <typename T, int S, int NW, int RB>
 inline int sor_launch_shape(const SorArgs<T>& a, bool gs, hipStream_t st) {
     const int ntiles = a.tiles_i * a.tiles_j;
-    if (gs)
-        hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, true>), dim3(ntiles), dim3(NW * 64), 0, st, a);
-    else
-        hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false>), dim3(ntiles), dim3(NW * 64), 0, st, a);
+    const dim3 grid(ntiles), block(NW * 64);
+    // FP32 always multiplies by RN(1/B); FP64 does when 1/B is exact
+    const bool xr = sizeof(T) == 4 || a.exact_rcp;
+    if (gs && xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, true, true>), grid, block, 0, st, a);
+    else if (gs) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, true, false>), grid, block, 0, st, a);
+    else if (xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, true>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, false>), grid, block, 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
