@@ -74,6 +74,16 @@ def lib():
         L.cnv_sweeps_per_launch.argtypes = [C.c_void_p]
         L.cnv_calibration_copy.argtypes = [C.c_void_p, C.c_longlong, C.POINTER(C.c_double)]
         L.cnv_device_name.argtypes = [C.c_int, C.c_char_p, C.c_int]
+        L.cnv_batch_create.argtypes = [C.POINTER(Params), C.c_int, C.POINTER(C.c_void_p)]
+        L.cnv_batch_destroy.argtypes = [C.c_void_p]
+        L.cnv_batch_size.argtypes = [C.c_void_p]
+        L.cnv_batch_last_error.argtypes = [C.c_void_p]
+        L.cnv_batch_last_error.restype = C.c_char_p
+        L.cnv_batch_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepStats)]
+        L.cnv_batch_step_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepStats), C.POINTER(C.c_double)]
+        L.cnv_batch_get_state.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
+        L.cnv_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -187,6 +197,100 @@ class Cavity:
     @property
     def sweeps_per_launch(self):
         return lib().cnv_sweeps_per_launch(self.h)
+
+
+class Batch:
+    """Many independent cavities advanced together on one GPU (a cnv_batch)."""
+
+    def __init__(self, params_list):
+        self.ns = [p.n for p in params_list]
+        arr = (Params * len(params_list))(*params_list)
+        h = C.c_void_p()
+        rc = lib().cnv_batch_create(arr, len(params_list), C.byref(h))
+        if rc != OK:
+            raise CnvError(rc, lib().cnv_batch_last_error(None).decode(errors="replace"))
+        self.h = h
+
+    def __len__(self):
+        return len(self.ns)
+
+    def _check(self, rc):
+        if rc != OK:
+            raise CnvError(rc, lib().cnv_batch_last_error(self.h).decode(errors="replace"))
+
+    def close(self):
+        if self.h:
+            lib().cnv_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps=1, check=True):
+        """Returns stats[t][i] for step t of cavity i."""
+        cnt = len(self.ns)
+        st = (StepStats * max(nsteps * cnt, 1))()
+        rc = lib().cnv_batch_step(self.h, nsteps, st)
+        if check:
+            self._check(rc)
+        flat = list(st)
+        return [flat[t * cnt:(t + 1) * cnt] for t in range(nsteps)]
+
+    def step_timed(self, nsteps):
+        cnt = len(self.ns)
+        st = (StepStats * max(nsteps * cnt, 1))()
+        ms = C.c_double()
+        self._check(lib().cnv_batch_step_timed(self.h, nsteps, st, C.byref(ms)))
+        flat = list(st)
+        return [flat[t * cnt:(t + 1) * cnt] for t in range(nsteps)], ms.value
+
+    def get_state(self, i):
+        n = self.ns[i]
+        out = [np.empty((n, n)) for _ in range(4)]
+        self._check(lib().cnv_batch_get_state(self.h, i, *[a.ctypes.data for a in out]))
+        return dict(zip(("w", "psi", "u", "v"), out))
+
+    def autotune(self, launches=20):
+        s, w, r, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._check(lib().cnv_batch_autotune(self.h, launches, C.byref(s), C.byref(w), C.byref(r),
+                                             C.byref(ms)))
+        return {"sweeps_per_launch": s.value, "tile_waves": w.value, "wave_rows": r.value,
+                "ms_per_sweep": ms.value}
+
+
+def batch_table(count=64, seed=0):
+    """The batched workload: `count` lid-driven cavities drawn with numpy's default_rng(seed):
+    side N uniform over {64, 96, 128, 192, 256, 384, 512}, Re log-uniform on [100, 5000], lid
+    velocity uniform on [0.5, 1.5]; dt = 0.2*min(h*h*Re/4, h) with h = 1/N; SOR relaxation from
+    the reference's rule for the grid side (src/main.c:134)."""
+    rng = np.random.default_rng(seed)
+    sides = np.array([64, 96, 128, 192, 256, 384, 512])
+    N = rng.choice(sides, size=count)
+    Re = np.exp(rng.uniform(np.log(100.0), np.log(5000.0), size=count))
+    U = rng.uniform(0.5, 1.5, size=count)
+    rows = []
+    for i in range(count):
+        n = int(N[i])
+        h = 1.0 / n
+        dt = 0.2 * min(h * h * float(Re[i]) / 4.0, h)
+        rows.append({"n": n, "Re": float(Re[i]), "U": float(U[i]), "dt": dt,
+                     "beta": reference_beta(n)})
+    return rows
+
+
+def batch_params(rows, precision=PREC_FP64, tol=1e-8, itmax=2000000, **kw):
+    """cnv_params for each row of batch_table (lid u4 = U, other walls at rest)."""
+    return [default_params(r["n"], Re=r["Re"], dt=r["dt"], beta=r["beta"], tol=tol, itmax=itmax,
+                           precision=precision, u=[0.0, 0.0, 0.0, r["U"]], **kw) for r in rows]
 
 
 def euler(w, dwdx, dwdy, d2x, d2y, u, v, Re, dt):

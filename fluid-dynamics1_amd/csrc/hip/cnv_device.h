@@ -85,6 +85,12 @@ struct SorArgs {
     int vec2;            // 1: ld is even and the buffers are 2*sizeof(T)-aligned (paired access)
 };
 
+// Batched SOR launch: workgroup -> (solve, tile); inst < 0 marks padding workgroups.
+struct SorBlock {
+    int inst;
+    int tile;
+};
+
 // ---- host launch wrappers (defined in the .hip files) ---------------------------------
 // SOR launch shape: s sweeps per launch, nw waves per workgroup holding rb rows each; the
 // workgroup's tile owns (nw*rb - 4s) rows x (128 - 4s) columns.

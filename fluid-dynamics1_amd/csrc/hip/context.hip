@@ -22,56 +22,18 @@
 
 #include "cnv.h"
 #include "cnv_device.h"
+#include "ctx.h"
 #include "../host/cnv_host.h"
 
 using namespace cnv;
 
 namespace {
 
-constexpr int kSlots = 4;  // pinned copies of the control block in flight
-
 char g_err[256] = "";
 
 }  // namespace
 
-struct cnv_ctx {
-    cnv_params p;
-    int n, ld;
-    size_t cells;
-    hipStream_t stream = nullptr;
-    // state
-    double* w[2] = {nullptr, nullptr};
-    int wcur = 0;
-    double* psi[2] = {nullptr, nullptr};
-    double* u = nullptr;
-    double* v = nullptr;
-    // tables
-    int* d_start = nullptr;
-    double* d_tab = nullptr;  // 4 operators x n x 7
-    float* d_ftab = nullptr;
-    Stencils st{};
-    // solver control
-    SorCtl* ctl = nullptr;    // the Poisson solve's two state slots
-    int fin_slot = 0;         // slot written by the last SOR launch of the last solve
-    Diag* diag = nullptr;
-    double* partial = nullptr;
-    SorCtl* h_ctl = nullptr;  // pinned, kSlots entries
-    Diag* h_diag = nullptr;   // pinned
-    hipEvent_t ev[kSlots] = {};
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    // defect correction
-    float* e32[2] = {nullptr, nullptr};
-    float* r32 = nullptr;
-    // launch shape
-    TileShape shape{8, 8, 10};
-    int tiles_i = 0, tiles_j = 0;
-    int launches_per_poll = 32;
-    int k_prev = 0;       // sweeps of the previous solve (queue-ahead prediction)
-    WallBC bc{};
-    char err[256] = "";
-};
-
-static int fail(cnv_ctx* c, int code, const char* fmt, ...) {
+int cnv_fail(cnv_ctx* c, int code, const char* fmt, ...) {
     char buf[256];
     va_list ap;
     va_start(ap, fmt);
@@ -81,18 +43,6 @@ static int fail(cnv_ctx* c, int code, const char* fmt, ...) {
     snprintf(g_err, sizeof(g_err), "%s", buf);
     return code;
 }
-
-#define HIPC(c, x)                                                                              \
-    do {                                                                                        \
-        hipError_t e_ = (x);                                                                    \
-        if (e_ != hipSuccess) return fail((c), CNV_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
-    } while (0)
-#define LAUNCH(c, x)                                                                           \
-    do {                                                                                        \
-        if ((x) != 0) return fail((c), CNV_ERR_HIP, "launch failed: %s (%s)", #x,             \
-                                  hipGetErrorString(hipGetLastError()));                        \
-    } while (0)
-
 // The public functions below are declared extern "C" in cnv.h, which gives them C linkage.
 
 double cnv_reference_beta(int nx, int ny) {
@@ -136,7 +86,7 @@ const char* cnv_last_error(const cnv_ctx* c) { return c ? c->err : g_err; }
 
 int cnv_device_name(int device, char* buf, int len) {
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(nullptr, CNV_ERR_NO_DEVICE, "no device %d", device);
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return cnv_fail(nullptr, CNV_ERR_NO_DEVICE, "no device %d", device);
     snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
     return CNV_OK;
 }
@@ -162,9 +112,9 @@ static int upload_tables(cnv_ctx* c) {
     const int which[4] = {1, 1, 2, 2};
     for (int t = 0; t < 4; ++t) {
         int rc = cnv_fd_dense(which[t], n, o, hs[t], D.data());
-        if (rc) return fail(c, CNV_ERR_ARG, "** Error: valid orders are 2, 4 or 6 ** (order=%d, n=%d)", o, n);
+        if (rc) return cnv_fail(c, CNV_ERR_ARG, "** Error: valid orders are 2, 4 or 6 ** (order=%d, n=%d)", o, n);
         if (cnv_fd_windows(n, D.data(), t == 0 ? start.data() : start2.data(), tab.data() + (size_t)t * n * kStencilWidth))
-            return fail(c, CNV_ERR_ARG, "stencil window does not cover a row (n=%d)", n);
+            return cnv_fail(c, CNV_ERR_ARG, "stencil window does not cover a row (n=%d)", n);
     }
     std::vector<float> ftab(tab.size());
     for (size_t q = 0; q < tab.size(); ++q) ftab[q] = (float)tab[q];
@@ -271,19 +221,19 @@ void cnv_destroy(cnv_ctx* c) {
 
 int cnv_create(const cnv_params* p, cnv_ctx** out) {
     *out = nullptr;
-    if (!p) return fail(nullptr, CNV_ERR_ARG, "null params");
-    if (p->n < 8) return fail(nullptr, CNV_ERR_ARG, "grid side n=%d: need n >= 8", p->n);
+    if (!p) return cnv_fail(nullptr, CNV_ERR_ARG, "null params");
+    if (p->n < 8) return cnv_fail(nullptr, CNV_ERR_ARG, "grid side n=%d: need n >= 8", p->n);
     if (p->order != 2 && p->order != 4 && p->order != 6)
-        return fail(nullptr, CNV_ERR_ARG, "** Error: valid orders are 2, 4 or 6 ** (order=%d)", p->order);
+        return cnv_fail(nullptr, CNV_ERR_ARG, "** Error: valid orders are 2, 4 or 6 ** (order=%d)", p->order);
     if (p->poisson_type != 1 && p->poisson_type != 2)
-        return fail(nullptr, CNV_ERR_ARG, "Error - invalid option for Poisson solver (poisson_type=%d)", p->poisson_type);
+        return cnv_fail(nullptr, CNV_ERR_ARG, "Error - invalid option for Poisson solver (poisson_type=%d)", p->poisson_type);
     if (p->precision != CNV_PREC_FP64 && p->precision != CNV_PREC_FP32_DC)
-        return fail(nullptr, CNV_ERR_ARG, "invalid precision %d", p->precision);
-    if (!(p->dx > 0) || !(p->dy > 0)) return fail(nullptr, CNV_ERR_ARG, "grid spacing must be positive");
+        return cnv_fail(nullptr, CNV_ERR_ARG, "invalid precision %d", p->precision);
+    if (!(p->dx > 0) || !(p->dy > 0)) return cnv_fail(nullptr, CNV_ERR_ARG, "grid spacing must be positive");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p->device || p->device < 0)
-        return fail(nullptr, CNV_ERR_NO_DEVICE, "HIP device %d not available (%d devices)", p->device, ndev);
-    if (hipSetDevice(p->device) != hipSuccess) return fail(nullptr, CNV_ERR_NO_DEVICE, "hipSetDevice(%d) failed", p->device);
+        return cnv_fail(nullptr, CNV_ERR_NO_DEVICE, "HIP device %d not available (%d devices)", p->device, ndev);
+    if (hipSetDevice(p->device) != hipSuccess) return cnv_fail(nullptr, CNV_ERR_NO_DEVICE, "hipSetDevice(%d) failed", p->device);
 
     cnv_ctx* c = new cnv_ctx();
     c->p = *p;
@@ -297,7 +247,7 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
     if (p->wave_rows > 0) c->shape.rb = p->wave_rows;
     if ((p->sweeps_per_launch && c->shape.s != p->sweeps_per_launch) || !sor_shape_supported(c->shape)) {
         cnv_destroy(c);
-        return fail(nullptr, CNV_ERR_ARG, "unsupported SOR launch shape (sweeps_per_launch %d, waves %d, rows per wave %d)",
+        return cnv_fail(nullptr, CNV_ERR_ARG, "unsupported SOR launch shape (sweeps_per_launch %d, waves %d, rows per wave %d)",
                     spl, p->tile_waves, p->wave_rows);
     }
     c->launches_per_poll = p->launches_per_poll > 0 ? p->launches_per_poll : 32;
@@ -345,44 +295,6 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
 int cnv_sweeps_per_launch(const cnv_ctx* c) { return c->shape.s; }
 
 // ---- SOR driver ---------------------------------------------------------------------------------
-// Paired (2-element vector) access needs an even row length and pitch and 2*sizeof(T)-aligned
-// buffers.
-template <typename T>
-static int sor_vec2(const SorArgs<T>& a) {
-    const uintptr_t m = 2 * sizeof(T) - 1;
-    return (a.nc % 2 == 0) && (a.ld % 2 == 0) && !((uintptr_t)a.src_f & m) && !((uintptr_t)a.psi0 & m) && !((uintptr_t)a.psi1 & m);
-}
-
-template <typename T>
-static SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int negate) {
-    SorArgs<T> a{};
-    a.src_f = src;
-    a.psi0 = b0;
-    a.psi1 = b1;
-    a.partial = c->partial;
-    a.ctl = c->ctl;
-    a.nr = c->n;
-    a.nc = c->n;
-    a.ld = c->ld;
-    a.tiles_i = c->tiles_i;
-    a.tiles_j = c->tiles_j;
-    a.negate = negate;
-    const double dx = c->p.dx, dy = c->p.dy;
-    // the reference evaluates dy*dy, dx*dx, dx*dx*dy*dy and 2*(dx*dx+dy*dy) left to right
-    a.c1 = (T)(dy * dy);
-    a.c2 = (T)(dx * dx);
-    a.c3 = (T)(dx * dx * dy * dy);
-    a.B = (T)(2 * (dx * dx + dy * dy));
-    a.beta = (T)c->p.beta;
-    a.omb = (T)(1 - c->p.beta);
-    a.rB = (T)(1.0 / (2 * (dx * dx + dy * dy)));
-    int e2 = 0;
-    const double Bd = 2 * (dx * dx + dy * dy);
-    a.exact_rcp = std::frexp(Bd, &e2) == 0.5 && std::isnormal(1.0 / Bd) && (1.0 / Bd) * Bd == 1.0;
-    a.vec2 = sor_vec2(a);
-    return a;
-}
-
 // Queue SOR launches until the device control block reports convergence or failure.  Keeps
 // one chunk queued behind the one being waited on, so the GPU never idles on the host.
 static int launch_sor(cnv_ctx* c, SorArgs<double> a, int L, bool gs) {
@@ -418,7 +330,7 @@ static int run_sor(cnv_ctx* c, const SorArgs<T>& a, SorCtl* final_ctl) {
         const int chunk = queued == 0 ? first : M;
         for (int l = 0; l < chunk; ++l) {
             if (launch_sor(c, a, (int)(queued + l), gs))
-                return fail(c, CNV_ERR_HIP, "SOR launch failed: %s", hipGetErrorString(hipGetLastError()));
+                return cnv_fail(c, CNV_ERR_HIP, "SOR launch failed: %s", hipGetErrorString(hipGetLastError()));
         }
         queued += chunk;
         c->fin_slot = (int)((queued - 1) & 1);
@@ -437,7 +349,7 @@ static int run_sor(cnv_ctx* c, const SorArgs<T>& a, SorCtl* final_ctl) {
                 return CNV_OK;
             }
         }
-        if (queued > cap + first) return fail(c, CNV_ERR_HIP, "SOR state did not terminate after %lld launches", queued);
+        if (queued > cap + first) return cnv_fail(c, CNV_ERR_HIP, "SOR state did not terminate after %lld launches", queued);
     }
 }
 
@@ -515,13 +427,13 @@ static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
     }
     if (stats) *stats = s;
     if (s.status != CNV_OK)
-        return fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
+        return cnv_fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
     return CNV_OK;
 }
 
 int cnv_step(cnv_ctx* c, int nsteps, cnv_step_stats* stats) {
-    if (!c || nsteps < 0) return fail(c, CNV_ERR_ARG, "bad arguments to cnv_step");
-    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    if (!c || nsteps < 0) return cnv_fail(c, CNV_ERR_ARG, "bad arguments to cnv_step");
+    if (hipSetDevice(c->p.device) != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipSetDevice failed");
     for (int t = 0; t < nsteps; ++t) {
         int rc = one_step(c, stats ? &stats[t] : nullptr);
         if (rc) return rc;
@@ -531,8 +443,8 @@ int cnv_step(cnv_ctx* c, int nsteps, cnv_step_stats* stats) {
 }
 
 int cnv_step_timed(cnv_ctx* c, int nsteps, cnv_step_stats* stats, double* elapsed_ms) {
-    if (!c || nsteps < 0) return fail(c, CNV_ERR_ARG, "bad arguments to cnv_step_timed");
-    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    if (!c || nsteps < 0) return cnv_fail(c, CNV_ERR_ARG, "bad arguments to cnv_step_timed");
+    if (hipSetDevice(c->p.device) != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipSetDevice failed");
     HIPC(c, hipStreamSynchronize(c->stream));
     HIPC(c, hipEventRecord(c->t0, c->stream));
     for (int t = 0; t < nsteps; ++t) {
@@ -548,7 +460,7 @@ int cnv_step_timed(cnv_ctx* c, int nsteps, cnv_step_stats* stats, double* elapse
 }
 
 int cnv_poisson_device(cnv_ctx* c, const double* d_f, double* d_psi, int* k, double* e) {
-    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    if (hipSetDevice(c->p.device) != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipSetDevice failed");
     SorCtl fin{};
     int rc = poisson_fp64(c, d_f, 0, &fin);
     if (rc) return rc;
@@ -557,12 +469,12 @@ int cnv_poisson_device(cnv_ctx* c, const double* d_f, double* d_psi, int* k, dou
     if (k) *k = fin.k - 1;
     if (e) *e = fin.e_last;
     if (fin.status != SOR_CONVERGED)
-        return fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
+        return cnv_fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
     return CNV_OK;
 }
 
 int cnv_poisson_host(cnv_ctx* c, const double* f, double* psi, int* k, double* e) {
-    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    if (hipSetDevice(c->p.device) != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipSetDevice failed");
     const size_t bytes = sizeof(double) * c->cells;
     // stage f in the spare w buffer: the solve reads it as its source term directly
     double* d_f = c->w[c->wcur ^ 1];
@@ -575,13 +487,13 @@ int cnv_poisson_host(cnv_ctx* c, const double* f, double* psi, int* k, double* e
     if (k) *k = fin.k - 1;
     if (e) *e = fin.e_last;
     if (fin.status != SOR_CONVERGED)
-        return fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
+        return cnv_fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
     return CNV_OK;
 }
 
 int cnv_time_sor(cnv_ctx* c, int launches, double* mean_ms) {
-    if (!c || launches <= 0) return fail(c, CNV_ERR_ARG, "bad arguments to cnv_time_sor");
-    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    if (!c || launches <= 0) return cnv_fail(c, CNV_ERR_ARG, "bad arguments to cnv_time_sor");
+    if (hipSetDevice(c->p.device) != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipSetDevice failed");
     const bool dc = c->p.precision == CNV_PREC_FP32_DC;
     // a solve that never converges (tol < 0) and never runs out of budget, on the current w
     if (dc) {
@@ -613,19 +525,19 @@ int cnv_time_sor(cnv_ctx* c, int launches, double* mean_ms) {
 
 int cnv_euler_host(int n, double* w, const double* dwdx, const double* dwdy, const double* d2wdx2,
                    const double* d2wdy2, const double* u, const double* v, double Re, double dt) {
-    if (n <= 0) return fail(nullptr, CNV_ERR_ARG, "bad size");
+    if (n <= 0) return cnv_fail(nullptr, CNV_ERR_ARG, "bad size");
     const size_t bytes = sizeof(double) * (size_t)n;
     double* d[7] = {};
     const double* h[7] = {w, dwdx, dwdy, d2wdx2, d2wdy2, u, v};
     int rc = CNV_OK;
     for (int q = 0; q < 7 && rc == CNV_OK; ++q) {
         if (hipMalloc(&d[q], bytes) != hipSuccess || hipMemcpy(d[q], h[q], bytes, hipMemcpyHostToDevice) != hipSuccess)
-            rc = fail(nullptr, CNV_ERR_HIP, "euler: device allocation/upload failed");
+            rc = cnv_fail(nullptr, CNV_ERR_HIP, "euler: device allocation/upload failed");
     }
     if (rc == CNV_OK) {
         if (euler_launch(d[0], d[1], d[2], d[3], d[4], d[5], d[6], 1. / Re, dt, n, nullptr) ||
             hipMemcpy(w, d[0], bytes, hipMemcpyDeviceToHost) != hipSuccess)
-            rc = fail(nullptr, CNV_ERR_HIP, "euler: kernel failed");
+            rc = cnv_fail(nullptr, CNV_ERR_HIP, "euler: kernel failed");
     }
     for (int q = 0; q < 7; ++q)
         if (d[q]) (void)hipFree(d[q]);
@@ -641,7 +553,7 @@ static int elementwise_host(int m, int n, const double* a, const double* b, doub
         hipMemcpy(da, a, bytes, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(db, b, bytes, hipMemcpyHostToDevice) != hipSuccess || add_launch(da, db, dc, (long long)cnt, sub, nullptr) ||
         hipMemcpy(out, dc, bytes, hipMemcpyDeviceToHost) != hipSuccess)
-        rc = fail(nullptr, CNV_ERR_HIP, "elementwise kernel failed");
+        rc = cnv_fail(nullptr, CNV_ERR_HIP, "elementwise kernel failed");
     if (da) (void)hipFree(da);
     if (db) (void)hipFree(db);
     if (dc) (void)hipFree(dc);
@@ -649,25 +561,25 @@ static int elementwise_host(int m, int n, const double* a, const double* b, doub
 }
 
 int cnv_add_host(int m, int n, const double* a, const double* b, double* out) {
-    if (m <= 0 || n <= 0) return fail(nullptr, CNV_ERR_ARG, "bad size");
+    if (m <= 0 || n <= 0) return cnv_fail(nullptr, CNV_ERR_ARG, "bad size");
     return elementwise_host(m, n, a, b, out, 0);
 }
 int cnv_sub_host(int m, int n, const double* a, const double* b, double* out) {
-    if (m <= 0 || n <= 0) return fail(nullptr, CNV_ERR_ARG, "bad size");
+    if (m <= 0 || n <= 0) return cnv_fail(nullptr, CNV_ERR_ARG, "bad size");
     return elementwise_host(m, n, a, b, out, 1);
 }
 
 int cnv_calibration_copy(cnv_ctx* c, long long nbytes, double* ms) {
-    if (!c || nbytes <= 0 || nbytes % 8) return fail(c, CNV_ERR_ARG, "calibration copy needs a positive multiple of 8 bytes");
-    if (hipSetDevice(c->p.device) != hipSuccess) return fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    if (!c || nbytes <= 0 || nbytes % 8) return cnv_fail(c, CNV_ERR_ARG, "calibration copy needs a positive multiple of 8 bytes");
+    if (hipSetDevice(c->p.device) != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipSetDevice failed");
     double *a = nullptr, *b = nullptr;
     HIPC(c, hipMalloc(&a, nbytes));
     HIPC(c, hipMalloc(&b, nbytes));
     int rc = CNV_OK;
-    if (copy_launch(b, a, nbytes / 8, c->stream)) rc = fail(c, CNV_ERR_HIP, "copy kernel failed");  // warm
+    if (copy_launch(b, a, nbytes / 8, c->stream)) rc = cnv_fail(c, CNV_ERR_HIP, "copy kernel failed");  // warm
     if (rc == CNV_OK && (hipEventRecord(c->t0, c->stream) != hipSuccess || copy_launch(b, a, nbytes / 8, c->stream) ||
                          hipEventRecord(c->t1, c->stream) != hipSuccess || hipEventSynchronize(c->t1) != hipSuccess))
-        rc = fail(c, CNV_ERR_HIP, "copy kernel failed");
+        rc = cnv_fail(c, CNV_ERR_HIP, "copy kernel failed");
     float t = 0.f;
     if (rc == CNV_OK && hipEventElapsedTime(&t, c->t0, c->t1) == hipSuccess && ms) *ms = t;
     (void)hipFree(a);
@@ -676,10 +588,10 @@ int cnv_calibration_copy(cnv_ctx* c, long long nbytes, double* ms) {
 }
 
 int cnv_set_poisson_params(cnv_ctx* c, double dx, double dy, double beta, double tol, int itmax, int poisson_type) {
-    if (!c) return fail(nullptr, CNV_ERR_ARG, "null context");
+    if (!c) return cnv_fail(nullptr, CNV_ERR_ARG, "null context");
     if (poisson_type != 1 && poisson_type != 2)
-        return fail(c, CNV_ERR_ARG, "Error - invalid option for Poisson solver (poisson_type=%d)", poisson_type);
-    if (!(dx > 0) || !(dy > 0)) return fail(c, CNV_ERR_ARG, "grid spacing must be positive");
+        return cnv_fail(c, CNV_ERR_ARG, "Error - invalid option for Poisson solver (poisson_type=%d)", poisson_type);
+    if (!(dx > 0) || !(dy > 0)) return cnv_fail(c, CNV_ERR_ARG, "grid spacing must be positive");
     c->p.dx = dx;
     c->p.dy = dy;
     c->p.beta = beta;

@@ -1,0 +1,104 @@
+// ctx.h -- internal to libcnv: the context structure behind the opaque cnv_ctx of cnv.h and
+// the helpers the one-cavity API (context.hip) and the batched API (batch.hip) share.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+
+#include "cnv.h"
+#include "cnv_device.h"
+
+constexpr int kSlots = 4;  // pinned copies of the control block in flight
+
+struct cnv_ctx {
+    cnv_params p;
+    int n, ld;
+    size_t cells;
+    hipStream_t stream = nullptr;
+    // state
+    double* w[2] = {nullptr, nullptr};
+    int wcur = 0;
+    double* psi[2] = {nullptr, nullptr};
+    double* u = nullptr;
+    double* v = nullptr;
+    // tables
+    int* d_start = nullptr;
+    double* d_tab = nullptr;  // 4 operators x n x 7
+    float* d_ftab = nullptr;
+    cnv::Stencils st{};
+    // solver control
+    cnv::SorCtl* ctl = nullptr;  // the Poisson solve's two state slots
+    int fin_slot = 0;            // slot written by the last SOR launch of the last solve
+    cnv::Diag* diag = nullptr;
+    double* partial = nullptr;
+    cnv::SorCtl* h_ctl = nullptr;  // pinned, kSlots entries
+    cnv::Diag* h_diag = nullptr;   // pinned
+    hipEvent_t ev[kSlots] = {};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    // defect correction
+    float* e32[2] = {nullptr, nullptr};
+    float* r32 = nullptr;
+    // launch shape
+    cnv::TileShape shape{8, 8, 10};
+    int tiles_i = 0, tiles_j = 0;
+    int launches_per_poll = 32;
+    int k_prev = 0;       // sweeps of the previous solve (queue-ahead prediction)
+    cnv::WallBC bc{};
+    char err[256] = "";
+};
+
+// Records the message (per context, and process-wide for cnv_last_error(NULL)); returns code.
+int cnv_fail(cnv_ctx* c, int code, const char* fmt, ...);
+
+#define HIPC(c, x)                                                                              \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) return cnv_fail((c), CNV_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+#define LAUNCH(c, x)                                                                           \
+    do {                                                                                        \
+        if ((x) != 0) return cnv_fail((c), CNV_ERR_HIP, "launch failed: %s (%s)", #x,         \
+                                      hipGetErrorString(hipGetLastError()));                    \
+    } while (0)
+
+// SOR launch arguments of context c for the planes src (source), b0 / b1 (ping-pong).
+// Paired (2-element vector) access needs an even row length and pitch and 2*sizeof(T)-aligned
+// buffers.
+template <typename T>
+inline int sor_vec2(const cnv::SorArgs<T>& a) {
+    const uintptr_t m = 2 * sizeof(T) - 1;
+    return (a.nc % 2 == 0) && (a.ld % 2 == 0) && !((uintptr_t)a.src_f & m) && !((uintptr_t)a.psi0 & m) && !((uintptr_t)a.psi1 & m);
+}
+
+template <typename T>
+inline cnv::SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int negate) {
+    cnv::SorArgs<T> a{};
+    a.src_f = src;
+    a.psi0 = b0;
+    a.psi1 = b1;
+    a.partial = c->partial;
+    a.ctl = c->ctl;
+    a.nr = c->n;
+    a.nc = c->n;
+    a.ld = c->ld;
+    a.tiles_i = c->tiles_i;
+    a.tiles_j = c->tiles_j;
+    a.negate = negate;
+    const double dx = c->p.dx, dy = c->p.dy;
+    // the reference evaluates dy*dy, dx*dx, dx*dx*dy*dy and 2*(dx*dx+dy*dy) left to right
+    a.c1 = (T)(dy * dy);
+    a.c2 = (T)(dx * dx);
+    a.c3 = (T)(dx * dx * dy * dy);
+    a.B = (T)(2 * (dx * dx + dy * dy));
+    a.beta = (T)c->p.beta;
+    a.omb = (T)(1 - c->p.beta);
+    a.rB = (T)(1.0 / (2 * (dx * dx + dy * dy)));
+    int e2 = 0;
+    const double Bd = 2 * (dx * dx + dy * dy);
+    a.exact_rcp = std::frexp(Bd, &e2) == 0.5 && std::isnormal(1.0 / Bd) && (1.0 / Bd) * Bd == 1.0;
+    a.vec2 = sor_vec2(a);
+    return a;
+}
+

@@ -19,6 +19,7 @@
 
 #include "cnv_device.h"
 #include "sor_rb.h"
+#include "step_kernels.h"
 
 namespace cnv {
 
@@ -34,14 +35,7 @@ __global__ void __launch_bounds__(kBlock) dc_residual_kernel(const double* __res
                                                             double B) {
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= n || j >= n) return;
-    const size_t q = (size_t)i * ld + j;
-    float r = 0.0f;
-    if (i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2) {
-        const double lap = (c1 * (psi[q + ld] + psi[q - ld]) + c2 * (psi[q + 1] + psi[q - 1]) - B * psi[q]) / c3;
-        r = (float)(-w[q] - lap);
-    }
-    r32[q] = r;
+    if (i < n && j < n) dc_residual_at(w, psi, r32, n, ld, c1, c2, c3, B, i, j);
 }
 
 // psi64 += e32 (the inner solve's final iterate, chosen by the control block's parity)
@@ -59,19 +53,7 @@ __global__ void __launch_bounds__(kBlock) dc_inner_init_kernel(float* e0, SorCtl
                                                               double rtol, double tol, int itmax) {
     const long long stride = (long long)gridDim.x * kBlock;
     for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) e0[q] = 0.0f;
-    if (blockIdx.x == 0 && threadIdx.x < 2) {  // both state slots; launch 0 reads slot 1
-        SorCtl* sl = ctl + threadIdx.x;
-        sl->k = 0;
-        sl->parity = 0;
-        sl->status = itmax > 0 ? SOR_RUNNING : SOR_FAILED;
-        sl->s_done = 0;
-        sl->launches = 0;
-        sl->itmax = itmax;
-        sl->e_last = 0.0;
-        sl->tol = tol;
-        sl->rtol = rtol;
-        sl->n_last = 0.0;
-    }
+    if (blockIdx.x == 0 && threadIdx.x < 2) ctl_reset(ctl + threadIdx.x, tol, rtol, itmax);
 }
 
 __global__ void __launch_bounds__(kBlock) zero_f32_kernel(float* p, long long count) {
@@ -79,34 +61,6 @@ __global__ void __launch_bounds__(kBlock) zero_f32_kernel(float* p, long long co
     if (q < count) p[q] = 0.0f;
 }
 
-// ---- FP32 stencils ---------------------------------------------------------------------------
-__device__ inline float bcf(const double* a, int i, int j, int n, int ld, double e0, double e1,
-                            double e2, double e3) {
-    if (j == 0) return (float)e0;
-    if (j == n - 1) return (float)e1;
-    if (i == 0) return (float)e2;
-    if (i == n - 1) return (float)e3;
-    return (float)a[(size_t)i * ld + j];
-}
-
-__device__ inline float fsum_x(const float* coef, const int* start, const double* f, int i, int j, int ld) {
-    const float* c = coef + (size_t)j * kStencilWidth;
-    const double* row = f + (size_t)i * ld + start[j];
-    float s = 0.0f;
-#pragma unroll
-    for (int m = 0; m < kStencilWidth; ++m) s += c[m] * (float)row[m];
-    return s;
-}
-__device__ inline float fsum_y(const float* coef, const int* start, const double* f, int i, int j, int ld) {
-    const float* c = coef + (size_t)i * kStencilWidth;
-    const double* col = f + (size_t)start[i] * ld + j;
-    float s = 0.0f;
-#pragma unroll
-    for (int m = 0; m < kStencilWidth; ++m) s += c[m] * (float)col[(size_t)m * ld];
-    return s;
-}
-
-// omega update: derivatives and right-hand side in FP32, the state update w + dt*rhs in FP64
 __global__ void __launch_bounds__(kBlock) transport_fp32_kernel(const double* __restrict__ w,
                                                                double* __restrict__ wout,
                                                                const double* __restrict__ u,
@@ -115,15 +69,7 @@ __global__ void __launch_bounds__(kBlock) transport_fp32_kernel(const double* __
                                                                double rRe, double dt) {
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= n || j >= n) return;
-    const float dwdx = fsum_x(s.f1x, s.start, w, i, j, ld);
-    const float dwdy = fsum_y(s.f1y, s.start, w, i, j, ld);
-    const float d2 = fsum_x(s.f2x, s.start, w, i, j, ld) + fsum_y(s.f2y, s.start, w, i, j, ld);
-    const float uu = bcf(u, i, j, n, ld, b.u1, b.u2, b.u3, b.u4);
-    const float vv = bcf(v, i, j, n, ld, b.v1, b.v2, b.v3, b.v4);
-    const float rhs = -uu * dwdx - vv * dwdy + (float)rRe * d2;
-    const size_t q = (size_t)i * ld + j;
-    wout[q] = w[q] + (double)rhs * dt;
+    if (i < n && j < n) transport_fp32_at(w, wout, u, v, n, ld, s, b, rRe, dt, i, j);
 }
 
 // u = DY psi, v = -(DX psi) with FP32 stencil arithmetic on the FP64 psi

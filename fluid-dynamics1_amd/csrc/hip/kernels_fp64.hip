@@ -3,78 +3,22 @@
 // Compiled with -ffp-contract=off (and the pragma below): every a*b+c stays two roundings, as in
 // the reference's x86-64 build, so that the stencil sums, the Euler update and the SOR update
 // reproduce the reference binary64 results bit for bit.
-//
-// Stencil sums: the reference multiplies dense Kronecker matrices by the flattened field
-// (src/main.c:142-152, src/linearalg.c:236-285): y[r] = 0.0 + A[r][0]*x[0] + A[r][1]*x[1] + ...
-// Adding the +-0 products of structural zeros never changes a sum, so the ascending sum over a
-// row's 7-column window (zero-padded, starting from 0.0) is the same number.
+// The per-point bodies live in step_kernels.h (shared with the batched kernels).
 #pragma clang fp contract(off)
 
 #include <float.h>
 
 #include "cnv_device.h"
 #include "sor_rb.h"
+#include "step_kernels.h"
 
 namespace cnv {
 
-// ---- velocity boundary values (src/main.c:283-296: rows first, columns win at corners) ----
-__device__ inline double bc_u(const double* u, int i, int j, int n, int ld, const WallBC& b) {
-    if (j == 0) return b.u1;
-    if (j == n - 1) return b.u2;
-    if (i == 0) return b.u3;
-    if (i == n - 1) return b.u4;
-    return u[(size_t)i * ld + j];
-}
-__device__ inline double bc_v(const double* v, int i, int j, int n, int ld, const WallBC& b) {
-    if (j == 0) return b.v1;
-    if (j == n - 1) return b.v2;
-    if (i == 0) return b.v3;
-    if (i == n - 1) return b.v4;
-    return v[(size_t)i * ld + j];
-}
-
-// (I (x) d) f at (i, j): sum over l of d[j][l] f[i][l]   (acts along j, i.e. x)
-__device__ inline double sum_x(const double* coef, const int* start, const double* f, int i, int j, int ld) {
-    const double* c = coef + (size_t)j * kStencilWidth;
-    const double* row = f + (size_t)i * ld + start[j];
-    double s = 0.0;
-#pragma unroll
-    for (int m = 0; m < kStencilWidth; ++m) s += c[m] * row[m];
-    return s;
-}
-// (d (x) I) f at (i, j): sum over k of d[i][k] f[k][j]   (acts along i, i.e. y)
-__device__ inline double sum_y(const double* coef, const int* start, const double* f, int i, int j, int ld) {
-    const double* c = coef + (size_t)i * kStencilWidth;
-    const double* col = f + (size_t)start[i] * ld + j;
-    double s = 0.0;
-#pragma unroll
-    for (int m = 0; m < kStencilWidth; ++m) s += c[m] * col[(size_t)m * ld];
-    return s;
-}
-
-// ---- wall vorticity: w[edge] = (Dx v - Dy u)[edge] on BC-applied u, v (src/main.c:298-320) --
 __global__ void __launch_bounds__(kBlock) wall_vorticity_kernel(double* w, const double* u, const double* v,
                                                                int n, int ld, Stencils s, WallBC b) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    int i, j;
-    if (t < n) { i = 0; j = t; }
-    else if (t < 2 * n) { i = n - 1; j = t - n; }
-    else if (t < 3 * n - 2) { i = t - 2 * n + 1; j = 0; }
-    else if (t < 4 * n - 4) { i = t - (3 * n - 2) + 1; j = n - 1; }
-    else return;
-    const double* c1x = s.d1x + (size_t)j * kStencilWidth;
-    const double* c1y = s.d1y + (size_t)i * kStencilWidth;
-    const int sj = s.start[j], si = s.start[i];
-    double dvdx = 0.0, dudy = 0.0;
-#pragma unroll
-    for (int m = 0; m < kStencilWidth; ++m) dvdx += c1x[m] * bc_v(v, i, sj + m, n, ld, b);
-#pragma unroll
-    for (int m = 0; m < kStencilWidth; ++m) dudy += c1y[m] * bc_u(u, si + m, j, n, ld, b);
-    w[(size_t)i * ld + j] = dvdx - dudy;
+    wall_vorticity_at(w, u, v, n, ld, s, b, blockIdx.x * kBlock + threadIdx.x);
 }
 
-// ---- derivatives of w and the explicit Euler update, every point (src/main.c:323-344,
-//      src/fluiddyn.c:98):  w' = (-u*dwdx - v*dwdy + (1./Re)*(d2wdx2+d2wdy2))*dt + w ---------
 __global__ void __launch_bounds__(kBlock) transport_fp64_kernel(const double* __restrict__ w,
                                                                double* __restrict__ wout,
                                                                const double* __restrict__ u,
@@ -83,14 +27,7 @@ __global__ void __launch_bounds__(kBlock) transport_fp64_kernel(const double* __
                                                                double rRe, double dt) {
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= n || j >= n) return;
-    const double dwdx = sum_x(s.d1x, s.start, w, i, j, ld);
-    const double dwdy = sum_y(s.d1y, s.start, w, i, j, ld);
-    const double d2wdx2 = sum_x(s.d2x, s.start, w, i, j, ld);
-    const double d2wdy2 = sum_y(s.d2y, s.start, w, i, j, ld);
-    const double uu = bc_u(u, i, j, n, ld, b);
-    const double vv = bc_v(v, i, j, n, ld, b);
-    wout[(size_t)i * ld + j] = (-uu * dwdx - vv * dwdy + rRe * (d2wdx2 + d2wdy2)) * dt + w[(size_t)i * ld + j];
+    if (i < n && j < n) transport_fp64_at(w, wout, u, v, n, ld, s, b, rRe, dt, i, j);
 }
 
 // ---- start of a Poisson solve: psi = 0 (src/poisson.c:229-230), reset the control block ----
@@ -98,26 +35,8 @@ __global__ void __launch_bounds__(kBlock) poisson_init_kernel(double* psi0, SorC
                                                              long long count, double tol, int itmax) {
     const long long stride = (long long)gridDim.x * kBlock;
     for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) psi0[q] = 0.0;
-    if (blockIdx.x == 0 && threadIdx.x < 2) {  // both state slots; launch 0 reads slot 1
-        SorCtl* sl = ctl + threadIdx.x;
-        sl->k = 0;
-        sl->parity = 0;
-        sl->status = itmax > 0 ? SOR_RUNNING : SOR_FAILED;
-        sl->s_done = 0;
-        sl->launches = 0;
-        sl->itmax = itmax;
-        sl->e_last = 0.0;
-        sl->tol = tol;
-        sl->rtol = 0.0;
-        sl->n_last = 0.0;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (diag) {
-            diag->kmax = dkey(-DBL_MAX);
-            diag->kmin = dkey(DBL_MAX);
-            diag->outer_norm = 0.0;
-        }
-    }
+    if (blockIdx.x == 0 && threadIdx.x < 2) ctl_reset(ctl + threadIdx.x, tol, 0.0, itmax);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && diag) diag_reset(diag);
 }
 
 // ---- velocities u = DY psi, v = -(DX psi) at every point (src/main.c:366-383) --------------
@@ -128,9 +47,7 @@ __global__ void __launch_bounds__(kBlock) velocity_kernel(const double* psi0, co
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= n || j >= n) return;
-    const double* psi = fin->parity ? psi1 : psi0;
-    u[(size_t)i * ld + j] = sum_y(s.d1y, s.start, psi, i, j, ld);
-    v[(size_t)i * ld + j] = -sum_x(s.d1x, s.start, psi, i, j, ld);
+    velocity_at(fin->parity ? psi1 : psi0, u, v, ld, s, i, j);
 }
 
 // ---- continuity diagnostic: max/min of DX u + DY v (src/main.c:387-408, fluiddyn.c:149,
@@ -143,27 +60,11 @@ __global__ void __launch_bounds__(kBlock) continuity_kernel(const double* __rest
     const long long stride = (long long)gridDim.x * kBlock;
     for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) {
         const int i = (int)(q / n), j = (int)(q - (long long)i * n);
-        const double c = sum_x(s.d1x, s.start, u, i, j, ld) + sum_y(s.d1y, s.start, v, i, j, ld);
+        const double c = continuity_at(u, v, ld, s, i, j);
         if (c > mx) mx = c;
         if (c < mn) mn = c;
     }
-    unsigned long long kx = dkey(mx), kn = dkey(mn);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long ox = __shfl_xor(kx, off, 64), on = __shfl_xor(kn, off, 64);
-        kx = ox > kx ? ox : kx;
-        kn = on < kn ? on : kn;
-    }
-    __shared__ unsigned long long sx[kBlock / 64], sn[kBlock / 64];
-    if ((threadIdx.x & 63) == 0) { sx[threadIdx.x >> 6] = kx; sn[threadIdx.x >> 6] = kn; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        kx = sx[0];
-        kn = sn[0];
-        for (int w = 1; w < kBlock / 64; ++w) { kx = sx[w] > kx ? sx[w] : kx; kn = sn[w] < kn ? sn[w] : kn; }
-        atomicMax(&diag->kmax, kx);
-        atomicMin(&diag->kmin, kn);
-    }
+    diag_fold_block(mx, mn, diag);
 }
 
 // ---- legacy elementwise entry points -------------------------------------------------------

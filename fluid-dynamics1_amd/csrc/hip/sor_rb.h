@@ -31,7 +31,7 @@
 // over owned points per workgroup in a fixed order into partial[L&1][s][tile].  The next launch
 // decides on them in its prologue, overlapped with its own (speculative) tile load: every
 // workgroup reduces the partials in tile order, so all reach the same decision, and workgroup 0
-// records it in the state slot L&1 (SorCtl) that the host polls:
+// (tile 0) records it in the state slot L&1 (SorCtl) that the host polls:
 //   * a launch whose previous state is terminal copies it forward and returns;
 //   * if sweep s < S-1 of the previous launch reached e < tol, that launch's output is
 //     discarded: this launch redoes s+1 sweeps from its (untouched) input buffer, so the final
@@ -125,8 +125,10 @@ __device__ inline float dpp_shift(float x) {
 constexpr int kDppFromNext = 0x130;  // wave_shl:1
 constexpr int kDppFromPrev = 0x138;  // wave_shr:1
 
+// One workgroup's launch on tile `tile` of the solve described by `a` (NW*64 threads).
 template <typename T, int S, int NW, int RB, bool GS, bool XR>
-__global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
+__device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile) {
+#pragma clang fp contract(off)
     constexpr int H = 2 * S;
     constexpr int EJ = 128;          // extended tile columns; lane m holds columns 2m, 2m+1
     constexpr int TJ = EJ - 2 * H;
@@ -147,7 +149,6 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
     SorCtl* mine = a.ctl + (L & 1);
     const int tid = threadIdx.x;
     const int ntiles = a.tiles_i * a.tiles_j;
-    const int tile = sor_tile_of_block(blockIdx.x, ntiles);
     const int ti = tile / a.tiles_j;
     const int tj = tile - ti * a.tiles_j;
     const int gi0 = ti * TI - H;  // even, so the colour of local (li, lj) is (li + lj) & 1
@@ -206,7 +207,7 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
 
     const int pstatus = prev->status;
     if (pstatus != SOR_RUNNING) {  // the solve already ended: carry its state forward
-        if (blockIdx.x == 0 && tid == 0) *mine = *prev;
+        if (tile == 0 && tid == 0) *mine = *prev;
         return;
     }
     const int sd = prev->s_done;   // sweeps the previous launch performed (0: first launch)
@@ -309,7 +310,7 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
     if (mode == 0) s_run = (itmax - next.k) < S ? (itmax - next.k) : S;
     next.s_done = (mode == 0) ? s_run : 0;
     if (mode != 1) next.launches += 1;
-    if (blockIdx.x == 0 && tid == 0) *mine = next;
+    if (tile == 0 && tid == 0) *mine = next;
     if (mode == 1) return;
     const int s_eff = s_run;
     // normal: read the previous launch's output, write the other buffer; rollback: read the
@@ -405,6 +406,23 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
     }
 }
 
+template <typename T, int S, int NW, int RB, bool GS, bool XR>
+__global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
+    sor_rb_body<T, S, NW, RB, GS, XR>(a, sor_tile_of_block(blockIdx.x, a.tiles_i * a.tiles_j));
+}
+
+// Batched form: workgroup b runs tile map[b].tile of solve map[b].inst (inst < 0: padding).
+// All solves of a batch share the launch index L.
+template <typename T, int S, int NW, int RB, bool GS, bool XR>
+__global__ void __launch_bounds__(NW * 64) sor_rb_batch_kernel(const SorArgs<T>* __restrict__ args,
+                                                               const SorBlock* __restrict__ map, int L) {
+    const SorBlock m = map[blockIdx.x];
+    if (m.inst < 0) return;
+    SorArgs<T> a = args[m.inst];
+    a.launch = L;
+    sor_rb_body<T, S, NW, RB, GS, XR>(a, m.tile);
+}
+
 template <typename T, int S, int NW, int RB>
 inline int sor_launch_shape(const SorArgs<T>& a, bool gs, hipStream_t st) {
     const int ntiles = a.tiles_i * a.tiles_j;
@@ -433,6 +451,27 @@ template <typename T>
 inline int sor_dispatch(const SorArgs<T>& a, TileShape sh, bool gs, hipStream_t st) {
 #define CNV_SOR_CASE(s_, nw_, rb_) \
     if (sh.s == s_ && sh.nw == nw_ && sh.rb == rb_) return sor_launch_shape<T, s_, nw_, rb_>(a, gs, st);
+    CNV_SOR_SHAPES(CNV_SOR_CASE)
+#undef CNV_SOR_CASE
+    return 2;
+}
+
+// Batched launch: FP64 uses the corrected division (exact for every B), FP32 the multiply.
+template <typename T, int S, int NW, int RB>
+inline int sor_batch_launch_shape(const SorArgs<T>* args, const SorBlock* map, int nblocks, int L, bool gs,
+                                  hipStream_t st) {
+    constexpr bool XR = sizeof(T) == 4;
+    const dim3 grid(nblocks), block(NW * 64);
+    if (gs) hipLaunchKernelGGL((sor_rb_batch_kernel<T, S, NW, RB, true, XR>), grid, block, 0, st, args, map, L);
+    else hipLaunchKernelGGL((sor_rb_batch_kernel<T, S, NW, RB, false, XR>), grid, block, 0, st, args, map, L);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+template <typename T>
+inline int sor_batch_dispatch(const SorArgs<T>* args, const SorBlock* map, int nblocks, int L, TileShape sh,
+                              bool gs, hipStream_t st) {
+#define CNV_SOR_CASE(s_, nw_, rb_) \
+    if (sh.s == s_ && sh.nw == nw_ && sh.rb == rb_) return sor_batch_launch_shape<T, s_, nw_, rb_>(args, map, nblocks, L, gs, st);
     CNV_SOR_SHAPES(CNV_SOR_CASE)
 #undef CNV_SOR_CASE
     return 2;

@@ -124,6 +124,29 @@ int cnv_sweeps_per_launch(const cnv_ctx *ctx);
 int cnv_calibration_copy(cnv_ctx *ctx, long long nbytes, double *ms);
 int cnv_device_name(int device, char *buf, int len);
 
+/* ---- batched cavities ------------------------------------------------------------------------
+ * Many independent cavities (own n, Re, dt, beta, tol, itmax, wall velocities) advanced together
+ * on one GPU: every phase of a time step is one launch covering all cavities, and each SOR
+ * launch advances every cavity whose solve is still running.  precision, order, poisson_type,
+ * device and the launch shape come from items[0] and must be the same in every item.  Each
+ * cavity's result equals the same cavity run alone (cnv_create / cnv_step) with the same launch
+ * shape, bit for bit.  No counterpart in the reference (one cavity per process there). */
+typedef struct cnv_batch cnv_batch;
+int cnv_batch_create(const cnv_params *items, int count, cnv_batch **out);
+void cnv_batch_destroy(cnv_batch *b);
+int cnv_batch_size(const cnv_batch *b);
+const char *cnv_batch_last_error(const cnv_batch *b);   /* b may be NULL: last batch error */
+/* Advance every cavity by nsteps.  stats: NULL or count*nsteps entries, stats[t*count + i]. */
+int cnv_batch_step(cnv_batch *b, int nsteps, cnv_step_stats *stats);
+int cnv_batch_step_timed(cnv_batch *b, int nsteps, cnv_step_stats *stats, double *elapsed_ms);
+/* State of cavity i (host arrays of n_i*n_i doubles; NULL pointers skip a field). */
+int cnv_batch_get_state(cnv_batch *b, int i, double *w, double *psi, double *u, double *v);
+/* Launch-shape autotuner: times `launches` batched SOR launches of every compiled shape on a
+ * synthetic source over the whole batch, keeps the fastest per sweep and reports it.  Leaves w,
+ * u, v as they were; the Poisson iterates are zeroed (every step recomputes psi from zero). */
+int cnv_batch_autotune(cnv_batch *b, int launches, int *sweeps_per_launch, int *tile_waves,
+                       int *wave_rows, double *ms_per_sweep);
+
 #ifdef __cplusplus
 }
 #endif

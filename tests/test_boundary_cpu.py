@@ -127,3 +127,23 @@ def test_bad_order_exits_like_reference(order):
     r = subprocess.run(["python3", "-c", code], capture_output=True, text=True)
     assert r.returncode == 1
     assert "** Error: valid orders are 2, 4 or 6 **" in r.stdout
+
+
+def test_batch_rejects_mixed_precision_without_gpu():
+    import cnv
+    ps = [cnv.default_params(32), cnv.default_params(40, precision=cnv.PREC_FP32_DC)]
+    with pytest.raises(cnv.CnvError) as ei:
+        cnv.Batch(ps)
+    assert ei.value.code == cnv.ERR_ARG and "must match item 0" in str(ei.value)
+
+
+def test_batch_workload_table_is_seed0_draw():
+    import cnv
+    rows = cnv.batch_table()
+    assert len(rows) == 64 and {r["n"] for r in rows} <= {64, 96, 128, 192, 256, 384, 512}
+    assert all(100.0 <= r["Re"] <= 5000.0 and 0.5 <= r["U"] <= 1.5 for r in rows)
+    for r in rows:  # dt = 0.2*min(h*h*Re/4, h), h = 1/N; reference relaxation rule
+        h = 1.0 / r["n"]
+        assert r["dt"] == 0.2 * min(h * h * r["Re"] / 4.0, h)
+        assert r["beta"] == cnv.reference_beta(r["n"])
+    assert rows == cnv.batch_table()
