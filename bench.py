@@ -92,6 +92,43 @@ def cpu_baseline(sample_seconds=12.0):
     return dict(t_transport=t_transport, t_sweep=t_sweep, sweeps=sweeps, threads=threads)
 
 
+def batched_leg(cnv, steps, precision, tag):
+    """The 64-cavity workload (seed 0) in one batch: autotuned launch shape, one warmup step, then
+    `steps` timed steps; beside it the same cavities stepped one at a time through the one-cavity
+    API (same warmup and steps), to show what batching buys."""
+    rows = cnv.batch_table()
+    cells = sum(r["n"] ** 2 for r in rows)
+    ps = cnv.batch_params(rows, precision=precision)
+    with cnv.Batch(ps) as b:
+        tune = b.autotune(launches=20)
+        print("bench: batch %s autotuned %s" % (tag, tune), file=sys.stderr, flush=True)
+        b.step(1)
+        st, ms = b.step_timed(steps)
+    sweeps = [sum(s.sweeps for s in row) for row in st]
+    print("bench: batch %s %.2f ms/step" % (tag, ms / steps), file=sys.stderr, flush=True)
+    alone_ms = 0.0
+    for p in ps:  # the same shape the autotuner chose, so the work per cavity is identical
+        p.sweeps_per_launch, p.tile_waves, p.wave_rows = tune["sweeps_per_launch"], tune["tile_waves"], tune["wave_rows"]
+        with cnv.Cavity(p) as cav:
+            cav.step(1)
+            _, t = cav.step_timed(steps)
+            alone_ms += t
+    print("bench: batch %s one at a time %.2f ms/step" % (tag, alone_ms / steps), file=sys.stderr, flush=True)
+    return {
+        "workload": "64 lid-driven cavities, seed 0: N in {64..512}, Re log-uniform [100, 5000], lid U in "
+                    "[0.5, 1.5], dt = 0.2*min(h^2 Re/4, h), reference beta per N, SOR tol 1e-8, order 6",
+        "cells": cells,
+        "value": cells * steps / (ms / 1e3) / 1e6,
+        "unit": "Mcell-steps/s",
+        "ms_per_step": ms / steps,
+        "steps": steps,
+        "autotuned": tune,
+        "sweeps_per_step_all_cavities": sweeps,
+        "one_at_a_time_ms_per_step": alone_ms / steps,
+        "speedup_vs_one_at_a_time": alone_ms / ms,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +140,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-dc", action="store_true", help="skip the FP32-refined leg")
     ap.add_argument("--sor-launches", type=int, default=400)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batched leg")
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--profile-sor", type=int, default=0, metavar="LAUNCHES",
                     help="profiling mode: one step, then LAUNCHES timed SOR launches and one 256 MiB "
                          "calibration copy (for counter scaling); prints no JSON")
@@ -178,6 +217,12 @@ def main():
         print("bench: dc %.3f ms/step, inner sweeps %s" % (ms32 / args.steps, dc["inner_sweeps_per_step"]),
               file=sys.stderr, flush=True)
 
+    # ---- batched 64-cavity workload -----------------------------------------------------------
+    batched = None
+    if not args.no_batch:
+        batched = batched_leg(cnv, args.batch_steps, cnv.PREC_FP32_DC, "fp32-refined")
+        batched["fp64"] = batched_leg(cnv, args.batch_steps, cnv.PREC_FP64, "fp64")
+
     # ---- CPU baseline -----------------------------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -212,6 +257,7 @@ def main():
                      "kernel": "sor_rb_kernel<double,...,S=%d>" % spl, "launch_ms": sor_ms},
         "cpu_baseline": cpu,
         "fp32_refined": dc,
+        "batched": batched,
         "device": dev,
     }
     print(json.dumps(line))

@@ -28,7 +28,7 @@ struct SorCtl {
     int itmax;          // sweep budget of this solve
     double e_last;      // L1 update of the last accepted sweep
     double tol;         // stop when a sweep's L1 update < tol
-    double rtol;        // fold when the L1 update < rtol * L1 norm of the iterate (0: never)
+    double rtol;        // stop ("fold") when the L1 update < rtol * L1 norm of the iterate (0: never)
     double n_last;      // L1 norm of the iterate after the last accepted sweep (FP32 planes)
 };
 
@@ -83,6 +83,7 @@ struct SorArgs {
     T rB;                // RN(1/B)
     int exact_rcp;       // 1: 1/B is exact (B a power of two), so x*rB == RN(x/B)
     int vec2;            // 1: ld is even and the buffers are 2*sizeof(T)-aligned (paired access)
+    int norm;            // FP64: also sum the iterate's L1 norm (needed when rtol > 0)
 };
 
 // Batched SOR launch: workgroup -> (solve, tile); inst < 0 marks padding workgroups.

@@ -295,64 +295,6 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
 int cnv_sweeps_per_launch(const cnv_ctx* c) { return c->shape.s; }
 
 // ---- SOR driver ---------------------------------------------------------------------------------
-// Queue SOR launches until the device control block reports convergence or failure.  Keeps
-// one chunk queued behind the one being waited on, so the GPU never idles on the host.
-static int launch_sor(cnv_ctx* c, SorArgs<double> a, int L, bool gs) {
-    a.launch = L;
-    return sor_fp64_launch(a, c->shape, gs, c->stream);
-}
-static int launch_sor(cnv_ctx* c, SorArgs<float> a, int L, bool gs) {
-    a.launch = L;
-    return sor_fp32_launch(a, c->shape, gs, c->stream);
-}
-
-static bool sor_done(int status) { return status != SOR_RUNNING; }
-
-// Queue SOR launches (launch index L = 0, 1, ...) until the state slot written by a queued
-// launch reports a terminal status.  Keeps one chunk queued behind the one being waited on, so
-// the GPU never idles on the host; launches queued after the end are no-ops.
-template <typename T>
-static int run_sor(cnv_ctx* c, const SorArgs<T>& a, SorCtl* final_ctl) {
-    const bool gs = (c->p.poisson_type == 1);
-    const int S = c->shape.s;
-    const int M = c->launches_per_poll;
-    const long long cap = (long long)c->p.itmax + 4LL * M + 8;  // working launches <= itmax/S + 2
-    int first = M;
-    if (c->k_prev > 0) {
-        const int predicted = c->k_prev / S - M;
-        if (predicted > first) first = predicted;
-    }
-    long long queued = 0;
-    int slot = 0;
-    int pending[kSlots];
-    int npend = 0;
-    while (true) {
-        const int chunk = queued == 0 ? first : M;
-        for (int l = 0; l < chunk; ++l) {
-            if (launch_sor(c, a, (int)(queued + l), gs))
-                return cnv_fail(c, CNV_ERR_HIP, "SOR launch failed: %s", hipGetErrorString(hipGetLastError()));
-        }
-        queued += chunk;
-        c->fin_slot = (int)((queued - 1) & 1);
-        HIPC(c, hipMemcpyAsync(&c->h_ctl[slot], c->ctl + c->fin_slot, sizeof(SorCtl), hipMemcpyDeviceToHost, c->stream));
-        HIPC(c, hipEventRecord(c->ev[slot], c->stream));
-        pending[npend++] = slot;
-        slot = (slot + 1) % kSlots;
-        if (npend >= 2) {
-            const int s0 = pending[0];
-            for (int q = 1; q < npend; ++q) pending[q - 1] = pending[q];
-            --npend;
-            HIPC(c, hipEventSynchronize(c->ev[s0]));
-            const SorCtl snap = c->h_ctl[s0];
-            if (sor_done(snap.status)) {
-                *final_ctl = snap;
-                return CNV_OK;
-            }
-        }
-        if (queued > cap + first) return cnv_fail(c, CNV_ERR_HIP, "SOR state did not terminate after %lld launches", queued);
-    }
-}
-
 static int poisson_fp64(cnv_ctx* c, const double* f_src, int negate, SorCtl* fin) {
     LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, c->p.tol, c->p.itmax, c->stream));
     SorArgs<double> a = sor_args<double>(c, f_src, c->psi[0], c->psi[1], negate);

@@ -102,3 +102,60 @@ inline cnv::SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int nega
     return a;
 }
 
+// Queue SOR launches until the device control block reports convergence or failure.  Keeps
+// one chunk queued behind the one being waited on, so the GPU never idles on the host.
+inline int launch_sor(cnv_ctx* c, cnv::SorArgs<double> a, int L, bool gs) {
+    a.launch = L;
+    return cnv::sor_fp64_launch(a, c->shape, gs, c->stream);
+}
+inline int launch_sor(cnv_ctx* c, cnv::SorArgs<float> a, int L, bool gs) {
+    a.launch = L;
+    return cnv::sor_fp32_launch(a, c->shape, gs, c->stream);
+}
+
+inline bool sor_done(int status) { return status != cnv::SOR_RUNNING; }
+
+// Queue SOR launches (launch index L = 0, 1, ...) until the state slot written by a queued
+// launch reports a terminal status.  Keeps one chunk queued behind the one being waited on, so
+// the GPU never idles on the host; launches queued after the end are no-ops.
+template <typename T>
+inline int run_sor(cnv_ctx* c, const cnv::SorArgs<T>& a, cnv::SorCtl* final_ctl) {
+    const bool gs = (c->p.poisson_type == 1);
+    const int S = c->shape.s;
+    const int M = c->launches_per_poll;
+    const long long cap = (long long)c->p.itmax + 4LL * M + 8;  // working launches <= itmax/S + 2
+    int first = M;
+    if (c->k_prev > 0) {
+        const int predicted = c->k_prev / S - M;
+        if (predicted > first) first = predicted;
+    }
+    long long queued = 0;
+    int slot = 0;
+    int pending[kSlots];
+    int npend = 0;
+    while (true) {
+        const int chunk = queued == 0 ? first : M;
+        for (int l = 0; l < chunk; ++l) {
+            if (launch_sor(c, a, (int)(queued + l), gs))
+                return cnv_fail(c, CNV_ERR_HIP, "SOR launch failed: %s", hipGetErrorString(hipGetLastError()));
+        }
+        queued += chunk;
+        c->fin_slot = (int)((queued - 1) & 1);
+        HIPC(c, hipMemcpyAsync(&c->h_ctl[slot], c->ctl + c->fin_slot, sizeof(cnv::SorCtl), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipEventRecord(c->ev[slot], c->stream));
+        pending[npend++] = slot;
+        slot = (slot + 1) % kSlots;
+        if (npend >= 2) {
+            const int s0 = pending[0];
+            for (int q = 1; q < npend; ++q) pending[q - 1] = pending[q];
+            --npend;
+            HIPC(c, hipEventSynchronize(c->ev[s0]));
+            const cnv::SorCtl snap = c->h_ctl[s0];
+            if (sor_done(snap.status)) {
+                *final_ctl = snap;
+                return CNV_OK;
+            }
+        }
+        if (queued > cap + first) return cnv_fail(c, CNV_ERR_HIP, "SOR state did not terminate after %lld launches", queued);
+    }
+}

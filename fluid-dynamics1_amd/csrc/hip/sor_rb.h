@@ -126,10 +126,13 @@ constexpr int kDppFromNext = 0x130;  // wave_shl:1
 constexpr int kDppFromPrev = 0x138;  // wave_shr:1
 
 // One workgroup's launch on tile `tile` of the solve described by `a` (NW*64 threads).
-template <typename T, int S, int NW, int RB, bool GS, bool XR>
+template <typename T, int S, int NW, int RB, bool GS, bool XR, bool NORM>
 __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile) {
 #pragma clang fp contract(off)
     constexpr int H = 2 * S;
+    // the iterate's L1 norm is needed by the relative stopping rule (rtol): always on FP32
+    // planes, on request (NORM) for FP64
+    constexpr bool kNorm = sizeof(T) == 4 || NORM;
     constexpr int EJ = 128;          // extended tile columns; lane m holds columns 2m, 2m+1
     constexpr int TJ = EJ - 2 * H;
     constexpr int EI = NW * RB;      // extended tile rows; wave w holds rows [w*RB, w*RB+RB)
@@ -227,7 +230,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 pv[s][0] += pp[((size_t)s * ntiles + t) * 2];
-                if (sizeof(T) == 4) pv[s][1] += pp[((size_t)s * ntiles + t) * 2 + 1];
+                if (kNorm) pv[s][1] += pp[((size_t)s * ntiles + t) * 2 + 1];
             }
         }
     }
@@ -242,8 +245,8 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
             }
         }
         __syncthreads();
-        // wave w reduces (sweep, quantity) rows; FP64 solves carry no iterate norm
-        const int nq = sizeof(T) == 4 ? 2 : 1;
+        // wave w reduces (sweep, quantity) rows; without kNorm there is no iterate norm
+        const int nq = kNorm ? 2 : 1;
         if (nq == 1 && tid < kMaxSweepsPerLaunch) etot[tid][1] = 0.0;
         for (int row = wv; row < nq * sd; row += NW) {
             const int s = row / nq, q = row - s * nq;
@@ -362,6 +365,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
                     cand = GS ? qt : qt + omb * old;
                     v[q][r] = cand;
                     acc[q] = __fma_rn(w[r], fabs(cand - old), acc[q]);
+                    if (kNorm) nrm[q] = __fma_rn(w[r], fabs(cand), nrm[q]);
                 } else {
                     const T x = __fmaf_rn(c1, up + dn, __fmaf_rn(c2, own + far, fc[q][r]));
                     const T y = be[q][r] * x;
@@ -377,7 +381,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         }
         // per-lane sums of this sweep, owned columns only; reduced once after the last sweep
         lred[s][0][tid] = (double)((ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0)));
-        if (sizeof(T) == 4)
+        if (kNorm)
             lred[s][1][tid] = (double)((ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0)));
     }
 
@@ -395,7 +399,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     if (mode != 0) return;  // a rollback launch ends the solve: no partials needed
     __syncthreads();
     // this launch's per-tile partials: wave w reduces (sweep, quantity) rows, in a fixed order
-    const int nq = sizeof(T) == 4 ? 2 : 1;
+    const int nq = kNorm ? 2 : 1;
     for (int row = wv; row < nq * s_eff; row += NW) {
         const int s = row / nq, q = row - s * nq;
         double acc = 0.0;
@@ -406,9 +410,9 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     }
 }
 
-template <typename T, int S, int NW, int RB, bool GS, bool XR>
+template <typename T, int S, int NW, int RB, bool GS, bool XR, bool NORM = false>
 __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
-    sor_rb_body<T, S, NW, RB, GS, XR>(a, sor_tile_of_block(blockIdx.x, a.tiles_i * a.tiles_j));
+    sor_rb_body<T, S, NW, RB, GS, XR, NORM>(a, sor_tile_of_block(blockIdx.x, a.tiles_i * a.tiles_j));
 }
 
 // Batched form: workgroup b runs tile map[b].tile of solve map[b].inst (inst < 0: padding).
@@ -420,7 +424,7 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_batch_kernel(const SorArgs<T>*
     if (m.inst < 0) return;
     SorArgs<T> a = args[m.inst];
     a.launch = L;
-    sor_rb_body<T, S, NW, RB, GS, XR>(a, m.tile);
+    sor_rb_body<T, S, NW, RB, GS, XR, false>(a, m.tile);
 }
 
 template <typename T, int S, int NW, int RB>
@@ -429,7 +433,10 @@ inline int sor_launch_shape(const SorArgs<T>& a, bool gs, hipStream_t st) {
     const dim3 grid(ntiles), block(NW * 64);
     // FP32 always multiplies by RN(1/B); FP64 does when 1/B is exact
     const bool xr = sizeof(T) == 4 || a.exact_rcp;
-    if (gs && xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, true, true>), grid, block, 0, st, a);
+    if (sizeof(T) == 8 && a.norm && !gs) {  // FP64 with the iterate norm (relative stopping rule)
+        if (xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, true, true>), grid, block, 0, st, a);
+        else hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, false, true>), grid, block, 0, st, a);
+    } else if (gs && xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, true, true>), grid, block, 0, st, a);
     else if (gs) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, true, false>), grid, block, 0, st, a);
     else if (xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, true>), grid, block, 0, st, a);
     else hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, false>), grid, block, 0, st, a);
