@@ -84,6 +84,13 @@ def lib():
         L.cnv_batch_get_state.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
         L.cnv_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.cnv_adj_create.argtypes = [C.POINTER(Params), C.c_int, C.c_double, C.c_double, C.POINTER(C.c_void_p)]
+        L.cnv_adj_destroy.argtypes = [C.c_void_p]
+        L.cnv_adj_last_error.argtypes = [C.c_void_p]
+        L.cnv_adj_last_error.restype = C.c_char_p
+        L.cnv_adj_forward.argtypes = [C.c_void_p, _DP, C.c_double, C.c_int, C.POINTER(C.c_double)]
+        L.cnv_adj_backward.argtypes = [C.c_void_p, _DP, C.POINTER(C.c_double)]
+        L.cnv_adj_sweeps.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         _lib = L
     return _lib
 
@@ -265,6 +272,74 @@ class Batch:
                                              C.byref(ms)))
         return {"sweeps_per_launch": s.value, "tile_waves": w.value, "wave_rows": r.value,
                 "ms_per_sweep": ms.value}
+
+
+class Adjoint:
+    """Gradient of the final kinetic energy w.r.t. the initial vorticity and the lid velocity
+    (a cnv_adjoint): forward(w0, U, nsteps) -> E, then backward() -> (dE/dw0, dE/dU)."""
+
+    def __init__(self, params, checkpoint_every=100, rtol_forward=1e-14, rtol_adjoint=1e-14):
+        self.n = params.n
+        h = C.c_void_p()
+        rc = lib().cnv_adj_create(C.byref(params), checkpoint_every, rtol_forward, rtol_adjoint, C.byref(h))
+        if rc != OK:
+            raise CnvError(rc, lib().cnv_adj_last_error(None).decode(errors="replace"))
+        self.h = h
+
+    def _check(self, rc):
+        if rc != OK:
+            raise CnvError(rc, lib().cnv_adj_last_error(self.h).decode(errors="replace"))
+
+    def close(self):
+        if self.h:
+            lib().cnv_adj_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, w0, U, nsteps):
+        w0 = np.ascontiguousarray(w0, dtype=np.float64)
+        assert w0.shape == (self.n, self.n)
+        E = C.c_double()
+        self._check(lib().cnv_adj_forward(self.h, w0, float(U), nsteps, C.byref(E)))
+        return E.value
+
+    def backward(self):
+        g = np.empty((self.n, self.n))
+        dU = C.c_double()
+        self._check(lib().cnv_adj_backward(self.h, g, C.byref(dU)))
+        return g, dU.value
+
+    def sweeps(self):
+        f, b = C.c_longlong(), C.c_longlong()
+        self._check(lib().cnv_adj_sweeps(self.h, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+
+def gaussian_vorticity(n, count=32, seed=1):
+    """Initial vorticity for the adjoint workload: a sum of `count` Gaussians drawn with numpy's
+    default_rng(seed) -- centres uniform on the unit square, widths uniform on [0.02, 0.1],
+    amplitudes N(0, 1) -- sampled at the nodes x = j/n, y = i/n (src/main.c:138-139)."""
+    rng = np.random.default_rng(seed)
+    cx, cy = rng.uniform(0.0, 1.0, count), rng.uniform(0.0, 1.0, count)
+    width = rng.uniform(0.02, 0.1, count)
+    amp = rng.standard_normal(count)
+    x = np.arange(n) / n
+    X, Y = np.meshgrid(x, x)  # X[i, j] = j/n, Y[i, j] = i/n
+    w = np.zeros((n, n))
+    for k in range(count):
+        w += amp[k] * np.exp(-((X - cx[k]) ** 2 + (Y - cy[k]) ** 2) / (2.0 * width[k] ** 2))
+    return w
 
 
 def batch_table(count=64, seed=0):

@@ -1,0 +1,510 @@
+// adjoint.hip -- gradient of the final kinetic energy of a cavity run with respect to the initial
+// vorticity field and the lid velocity: hand-written adjoint kernels of the full FP64 time step
+// with checkpoint-and-recompute.
+//
+// The step (src/main.c:276-451) carries the state s_t = (w_t, u_t, v_t); psi is recomputed from w
+// every step.  With ub, vb the BC-applied velocities (src/main.c:283-296; the lid row i = n-1,
+// columns 1..n-2, holds U), it is
+//   wt      = w_t with its edges replaced by (Dx vb - Dy ub)             (src/main.c:298-320)
+//   w_{t+1} = wt + dt*(-ub.(Dx wt) - vb.(Dy wt) + (1/Re)(Dxx wt + Dyy wt)) (src/fluiddyn.c:98)
+//   psi     = A^-1 (c3 w_{t+1}) on the interior, 0 on the edges: A is the operator whose fixed
+//             point the SOR iteration reaches, A psi = B psi - c1(N+S) - c2(E+W)
+//             (src/poisson.c:268, f = -w)
+//   u_{t+1} = Dy psi,  v_{t+1} = -(Dx psi)                               (src/main.c:366-383)
+// and the objective is E = 1/2 dx dy sum over all points of (u_T^2 + v_T^2).
+// The reverse step, given the adjoints (lw, lu, lv) of s_{t+1}:
+//   lpsi = Dy^T lu - Dx^T lv, interior only
+//   lw  += c3 A^-1 lpsi        (A is symmetric: the same SOR solve, the self-adjoint Poisson step)
+//   lwt  = lw + dt*(-Dx^T(ub.lw) - Dy^T(vb.lw) + (1/Re)(Dxx^T lw + Dyy^T lw))
+//   lub  = -dt (Dx wt).lw,  lvb = -dt (Dy wt).lw
+//   lvb += Dx^T(edge(lwt)),  lub -= Dy^T(edge(lwt))              (wall vorticity)
+//   (lw, lu, lv) of s_t = interior parts of (lwt, lub, lvb);  dE/dU += sum of lub on the lid
+// The transposed operators use windowed tables built on the host from the same dense Diff
+// matrices (exact transposes).  The forward steps stop their Poisson solves on a relative rule
+// (L1 update < rtol * L1 norm of the iterate) so that the recomputed trajectory, the adjoint solves
+// and finite differences all see fully converged solves; an absolute 1e-14 update is below the
+// resolution of an FP64 sum over the grid.
+//
+// Checkpoints: the forward pass stores s_t every K steps in device memory; the backward pass
+// recomputes each K-step segment from its checkpoint (storing its states) and runs the reverse
+// steps through it, last segment first.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "cnv.h"
+#include "cnv_device.h"
+#include "ctx.h"
+#include "step_kernels.h"
+#include "../host/cnv_host.h"
+
+using namespace cnv;
+
+namespace {
+
+constexpr int kTW = 10;  // transposed windows: Diff1/Diff2 columns span at most 10 rows
+
+struct TStencils {
+    const int* start;   // start[l]: first row of column l's window
+    const double* d1x;  // [l][m] = Diff1(dx)[start[l] + m][l]
+    const double* d1y;
+    const double* d2x;
+    const double* d2y;
+};
+
+// (Dx^T g)[i][l] = sum_j d1x[j][l] g[i][j];  optional pointwise weight p on g (ub) and edge mask
+template <int MODE>  // 0: g, 1: ub.g, 2: edge(g)
+__device__ inline double tsum_x(const double* cT, const TStencils& T, const double* g, const double* u, int n,
+                                int ld, const WallBC& bc, int i, int l) {
+    const double* c = cT + (size_t)l * kTW;
+    const int j0 = T.start[l];
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < kTW; ++m) {
+        const int j = j0 + m;
+        double x = g[(size_t)i * ld + j];
+        if (MODE == 1) x *= bc_u(u, i, j, n, ld, bc);
+        if (MODE == 2 && !(i == 0 || i == n - 1 || j == 0 || j == n - 1)) x = 0.0;
+        s += c[m] * x;
+    }
+    return s;
+}
+// (Dy^T g)[k][j] = sum_i d1y[i][k] g[i][j]
+template <int MODE>  // 0: g, 1: vb.g, 2: edge(g)
+__device__ inline double tsum_y(const double* cT, const TStencils& T, const double* g, const double* v, int n,
+                                int ld, const WallBC& bc, int k, int j) {
+    const double* c = cT + (size_t)k * kTW;
+    const int i0 = T.start[k];
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < kTW; ++m) {
+        const int i = i0 + m;
+        double x = g[(size_t)i * ld + j];
+        if (MODE == 1) x *= bc_v(v, i, j, n, ld, bc);
+        if (MODE == 2 && !(i == 0 || i == n - 1 || j == 0 || j == n - 1)) x = 0.0;
+        s += c[m] * x;
+    }
+    return s;
+}
+
+__device__ inline bool cell(int n, int& i, int& j) {
+    j = blockIdx.x * 64 + (threadIdx.x & 63);
+    i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    return i < n && j < n;
+}
+__device__ inline bool interior(int n, int i, int j) { return i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2; }
+
+// seed of the objective: lu = dxdy u_T, lv = dxdy v_T, lw = 0
+__global__ void __launch_bounds__(kBlock) adj_seed_kernel(const double* u, const double* v, double* lw, double* lu,
+                                                         double* lv, int n, int ld, double h2) {
+    int i, j;
+    if (!cell(n, i, j)) return;
+    const size_t q = (size_t)i * ld + j;
+    lw[q] = 0.0;
+    lu[q] = h2 * u[q];
+    lv[q] = h2 * v[q];
+}
+
+// adjoint Poisson source: src = (Dy^T lu - Dx^T lv) / c3 on the interior, 0 on the edges; the
+// SOR solve with f = -src reaches A z = lpsi
+__global__ void __launch_bounds__(kBlock) adj_vel_src_kernel(const double* lu, const double* lv, double* src, int n,
+                                                            int ld, TStencils T, WallBC bc, double c3) {
+    int i, j;
+    if (!cell(n, i, j)) return;
+    double s = 0.0;
+    if (interior(n, i, j))
+        s = (tsum_y<0>(T.d1y, T, lu, nullptr, n, ld, bc, i, j) - tsum_x<0>(T.d1x, T, lv, nullptr, n, ld, bc, i, j)) / c3;
+    src[(size_t)i * ld + j] = s;
+}
+
+// lw += c3 z on the interior (z: the adjoint solve's final iterate)
+__global__ void __launch_bounds__(kBlock) adj_accum_kernel(double* lw, const double* z0, const double* z1,
+                                                          const SorCtl* fin, int n, int ld, double c3) {
+    int i, j;
+    if (!cell(n, i, j) || !interior(n, i, j)) return;
+    const double* z = fin->parity ? z1 : z0;
+    const size_t q = (size_t)i * ld + j;
+    lw[q] += c3 * z[q];
+}
+
+// transport adjoint: lwt, lub, lvb from lw, wt and the state's velocities
+__global__ void __launch_bounds__(kBlock) adj_transport_kernel(const double* lw, const double* wt, const double* u,
+                                                              const double* v, double* lwt, double* lub, double* lvb,
+                                                              int n, int ld, Stencils S, TStencils T, WallBC bc,
+                                                              double rRe, double dt) {
+    int i, l;
+    if (!cell(n, i, l)) return;
+    const size_t q = (size_t)i * ld + l;
+    const double adv = -tsum_x<1>(T.d1x, T, lw, u, n, ld, bc, i, l) - tsum_y<1>(T.d1y, T, lw, v, n, ld, bc, i, l);
+    const double dif = tsum_x<0>(T.d2x, T, lw, nullptr, n, ld, bc, i, l) + tsum_y<0>(T.d2y, T, lw, nullptr, n, ld, bc, i, l);
+    lwt[q] = lw[q] + dt * (adv + rRe * dif);
+    lub[q] = -dt * sum_x(S.d1x, S.start, wt, i, l, ld) * lw[q];
+    lvb[q] = -dt * sum_y(S.d1y, S.start, wt, i, l, ld) * lw[q];
+}
+
+// wall-vorticity adjoint: lvb += Dx^T(edge(lwt)), lub -= Dy^T(edge(lwt))
+__global__ void __launch_bounds__(kBlock) adj_wall_kernel(const double* lwt, double* lub, double* lvb, int n, int ld,
+                                                         TStencils T, WallBC bc) {
+    int i, l;
+    if (!cell(n, i, l)) return;
+    const size_t q = (size_t)i * ld + l;
+    lvb[q] += tsum_x<2>(T.d1x, T, lwt, nullptr, n, ld, bc, i, l);
+    lub[q] -= tsum_y<2>(T.d1y, T, lwt, nullptr, n, ld, bc, i, l);
+}
+
+// adjoint of s_t: interior parts (the BCs overwrite the edges of u, v and the wall vorticity those
+// of w before they are read)
+__global__ void __launch_bounds__(kBlock) adj_finalize_kernel(const double* lwt, const double* lub, const double* lvb,
+                                                             double* lw, double* lu, double* lv, int n, int ld) {
+    int i, j;
+    if (!cell(n, i, j)) return;
+    const size_t q = (size_t)i * ld + j;
+    const bool in = interior(n, i, j);
+    lw[q] = in ? lwt[q] : 0.0;
+    lu[q] = in ? lub[q] : 0.0;
+    lv[q] = in ? lvb[q] : 0.0;
+}
+
+// dE/dU += sum of lub over the lid (row n-1, columns 1..n-2), in column order (one thread)
+__global__ void adj_lid_kernel(const double* lub, double* dU, int n, int ld) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int j = 1; j <= n - 2; ++j) s += lub[(size_t)(n - 1) * ld + j];
+    *dU += s;
+}
+
+// both state slots of a solve that stops on the relative rule
+__global__ void adj_slots_kernel(SorCtl* ctl, double tol, double rtol, int itmax) {
+    if (threadIdx.x < 2) ctl_reset(ctl + threadIdx.x, tol, rtol, itmax);
+}
+
+__global__ void adj_zero_kernel(double* p, long long count) {
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q < count) p[q] = 0.0;
+}
+
+char g_aerr[256] = "";
+
+}  // namespace
+
+struct cnv_adjoint {
+    cnv_ctx* c = nullptr;
+    int K = 100;
+    double rtol_fwd = 1e-14, rtol_adj = 1e-14;
+    int T = 0;  // steps of the last forward run
+    double U = 1.0;
+    size_t cells = 0;
+    // transposed tables
+    int* d_tstart = nullptr;
+    double* d_ttab = nullptr;
+    TStencils ts{};
+    // storage
+    double* d_ckpt = nullptr;  // [nckpt][3][cells]
+    int nckpt_cap = 0;
+    double* d_seg = nullptr;   // [K][3][cells]
+    double *lw = nullptr, *lu = nullptr, *lv = nullptr, *lwt = nullptr, *lub = nullptr, *lvb = nullptr;
+    double *wt = nullptr, *src = nullptr, *dU = nullptr;
+    int k_fwd = 0, k_adj = 0;  // queue-ahead predictions of the two kinds of solve
+    long long sweeps_fwd = 0, sweeps_adj = 0;
+    char err[256] = "";
+};
+
+static int afail(cnv_adjoint* a, int code, const char* fmt, ...) {
+    char buf[256];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (a) snprintf(a->err, sizeof(a->err), "%s", buf);
+    snprintf(g_aerr, sizeof(g_aerr), "%s", buf);
+    return code;
+}
+#define AHIP(a, x)                                                                                    \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) return afail((a), CNV_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+#define ALAUNCH(a)                                                                                    \
+    do {                                                                                              \
+        hipError_t e_ = hipGetLastError();                                                            \
+        if (e_ != hipSuccess) return afail((a), CNV_ERR_HIP, "adjoint launch failed: %s", hipGetErrorString(e_)); \
+    } while (0)
+#define ACTX(a, x)                                                                                    \
+    do {                                                                                              \
+        const int rc_ = (x);                                                                          \
+        if (rc_) return afail((a), rc_, "%s", (a)->c->err);                                          \
+    } while (0)
+
+static dim3 cell_grid(int n) { return dim3((n + 63) / 64, (n + 3) / 4); }
+
+// Poisson solve on the context's grid with the relative stopping rule: psi[parity] <- A^-1(...)
+// for the source src read as f = -src.  Returns the sweeps taken.
+static int solve_rel(cnv_adjoint* a, const double* src, double rtol, int* kpred, int* sweeps) {
+    cnv_ctx* c = a->c;
+    ACTX(a, poisson_init_launch(c->psi[0], c->ctl, nullptr, c->n, c->ld, 0.0, c->p.itmax, c->stream));
+    hipLaunchKernelGGL(adj_slots_kernel, dim3(1), dim3(64), 0, c->stream, c->ctl, 1e-300, rtol, c->p.itmax);
+    ALAUNCH(a);
+    SorArgs<double> sa = sor_args<double>(c, src, c->psi[0], c->psi[1], 1);
+    sa.norm = 1;
+    SorCtl fin{};
+    c->k_prev = *kpred;
+    ACTX(a, run_sor<double>(c, sa, &fin));
+    if (fin.status == SOR_FAILED)
+        return afail(a, CNV_NOT_CONVERGED, "adjoint-problem Poisson solve did not converge in %d sweeps", fin.k);
+    *kpred = fin.k;
+    *sweeps = fin.k;
+    return CNV_OK;
+}
+
+// one forward step on the context state (w[wcur], u, v), Poisson solve on the relative rule
+static int fwd_step(cnv_adjoint* a) {
+    cnv_ctx* c = a->c;
+    const int n = c->n, ld = c->ld;
+    double* w_in = c->w[c->wcur];
+    double* w_out = c->w[c->wcur ^ 1];
+    ACTX(a, wall_vorticity_launch(w_in, c->u, c->v, n, ld, c->st, c->bc, c->stream));
+    ACTX(a, transport_fp64_launch(w_in, w_out, c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re, c->p.dt, c->stream));
+    c->wcur ^= 1;
+    int k = 0;
+    int rc = solve_rel(a, w_out, a->rtol_fwd, &a->k_fwd, &k);
+    if (rc) return rc;
+    a->sweeps_fwd += k;
+    ACTX(a, velocity_launch(c->psi[0], c->psi[1], c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
+    return CNV_OK;
+}
+
+static double* slot3(double* base, int idx, int f, size_t cells) { return base + ((size_t)idx * 3 + f) * cells; }
+
+static int save_state(cnv_adjoint* a, double* base, int idx) {
+    cnv_ctx* c = a->c;
+    const size_t bytes = sizeof(double) * a->cells;
+    AHIP(a, hipMemcpyAsync(slot3(base, idx, 0, a->cells), c->w[c->wcur], bytes, hipMemcpyDeviceToDevice, c->stream));
+    AHIP(a, hipMemcpyAsync(slot3(base, idx, 1, a->cells), c->u, bytes, hipMemcpyDeviceToDevice, c->stream));
+    AHIP(a, hipMemcpyAsync(slot3(base, idx, 2, a->cells), c->v, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return CNV_OK;
+}
+static int load_state(cnv_adjoint* a, double* base, int idx) {
+    cnv_ctx* c = a->c;
+    const size_t bytes = sizeof(double) * a->cells;
+    AHIP(a, hipMemcpyAsync(c->w[c->wcur], slot3(base, idx, 0, a->cells), bytes, hipMemcpyDeviceToDevice, c->stream));
+    AHIP(a, hipMemcpyAsync(c->u, slot3(base, idx, 1, a->cells), bytes, hipMemcpyDeviceToDevice, c->stream));
+    AHIP(a, hipMemcpyAsync(c->v, slot3(base, idx, 2, a->cells), bytes, hipMemcpyDeviceToDevice, c->stream));
+    return CNV_OK;
+}
+
+// reverse of one step whose input state is stored at seg slot idx; (lw, lu, lv) in/out
+static int rev_step(cnv_adjoint* a, int idx) {
+    cnv_ctx* c = a->c;
+    const int n = c->n, ld = c->ld;
+    const double dx = c->p.dx, dy = c->p.dy;
+    const double c3 = dx * dx * dy * dy;
+    const dim3 g = cell_grid(n), blk(kBlock);
+    hipLaunchKernelGGL(adj_vel_src_kernel, g, blk, 0, c->stream, a->lu, a->lv, a->src, n, ld, a->ts, c->bc, c3);
+    ALAUNCH(a);
+    int k = 0;
+    int rc = solve_rel(a, a->src, a->rtol_adj, &a->k_adj, &k);
+    if (rc) return rc;
+    a->sweeps_adj += k;
+    hipLaunchKernelGGL(adj_accum_kernel, g, blk, 0, c->stream, a->lw, c->psi[0], c->psi[1], c->ctl + c->fin_slot, n,
+                       ld, c3);
+    ALAUNCH(a);
+    const double* ws = slot3(a->d_seg, idx, 0, a->cells);
+    const double* us = slot3(a->d_seg, idx, 1, a->cells);
+    const double* vs = slot3(a->d_seg, idx, 2, a->cells);
+    AHIP(a, hipMemcpyAsync(a->wt, ws, sizeof(double) * a->cells, hipMemcpyDeviceToDevice, c->stream));
+    ACTX(a, wall_vorticity_launch(a->wt, us, vs, n, ld, c->st, c->bc, c->stream));
+    hipLaunchKernelGGL(adj_transport_kernel, g, blk, 0, c->stream, a->lw, a->wt, us, vs, a->lwt, a->lub, a->lvb, n, ld,
+                       c->st, a->ts, c->bc, 1. / c->p.Re, c->p.dt);
+    ALAUNCH(a);
+    hipLaunchKernelGGL(adj_wall_kernel, g, blk, 0, c->stream, a->lwt, a->lub, a->lvb, n, ld, a->ts, c->bc);
+    ALAUNCH(a);
+    hipLaunchKernelGGL(adj_lid_kernel, dim3(1), dim3(64), 0, c->stream, a->lub, a->dU, n, ld);
+    ALAUNCH(a);
+    hipLaunchKernelGGL(adj_finalize_kernel, g, blk, 0, c->stream, a->lwt, a->lub, a->lvb, a->lw, a->lu, a->lv, n, ld);
+    ALAUNCH(a);
+    return CNV_OK;
+}
+
+static int build_transposed(cnv_adjoint* a) {
+    cnv_ctx* c = a->c;
+    const int n = c->n, o = c->p.order;
+    std::vector<int> start(n), tstart(n);
+    std::vector<double> D((size_t)n * n), win((size_t)n * kStencilWidth);
+    // structural windows (every operator has the same ones): row r covers [start[r], start[r]+7)
+    if (cnv_fd_dense(1, n, o, c->p.dx, D.data()) || cnv_fd_windows(n, D.data(), start.data(), win.data()))
+        return afail(a, CNV_ERR_ARG, "bad stencil order %d for n = %d", o, n);
+    for (int l = 0; l < n; ++l) {
+        int jlo = n, jhi = -1;
+        for (int j = 0; j < n; ++j)
+            if (start[j] <= l && l < start[j] + kStencilWidth) {
+                jlo = std::min(jlo, j);
+                jhi = std::max(jhi, j);
+            }
+        if (jhi - jlo + 1 > kTW) return afail(a, CNV_ERR_ARG, "transposed stencil wider than %d rows", kTW);
+        tstart[l] = std::min(jlo, n - kTW);
+    }
+    std::vector<double> tab((size_t)4 * n * kTW);
+    const double hs[4] = {c->p.dx, c->p.dy, c->p.dx, c->p.dy};
+    const int which[4] = {1, 1, 2, 2};
+    for (int t = 0; t < 4; ++t) {
+        if (cnv_fd_dense(which[t], n, o, hs[t], D.data())) return afail(a, CNV_ERR_ARG, "bad stencil order");
+        for (int l = 0; l < n; ++l)
+            for (int m = 0; m < kTW; ++m) tab[((size_t)t * n + l) * kTW + m] = D[(size_t)(tstart[l] + m) * n + l];
+    }
+    AHIP(a, hipMalloc(&a->d_tstart, sizeof(int) * n));
+    AHIP(a, hipMalloc(&a->d_ttab, sizeof(double) * tab.size()));
+    AHIP(a, hipMemcpy(a->d_tstart, tstart.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    AHIP(a, hipMemcpy(a->d_ttab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    const size_t stride = (size_t)n * kTW;
+    a->ts = TStencils{a->d_tstart, a->d_ttab, a->d_ttab + stride, a->d_ttab + 2 * stride, a->d_ttab + 3 * stride};
+    return CNV_OK;
+}
+
+// ---- C ABI -----------------------------------------------------------------------------------
+
+const char* cnv_adj_last_error(const cnv_adjoint* a) { return a ? a->err : g_aerr; }
+
+void cnv_adj_destroy(cnv_adjoint* a) {
+    if (!a) return;
+    if (a->c) {
+        (void)hipStreamSynchronize(a->c->stream);
+        cnv_destroy(a->c);
+    }
+    void* dev[] = {a->d_tstart, a->d_ttab, a->d_ckpt, a->d_seg, a->lw, a->lu, a->lv, a->lwt, a->lub, a->lvb,
+                   a->wt, a->src, a->dU};
+    for (void* p : dev)
+        if (p) (void)hipFree(p);
+    delete a;
+}
+
+int cnv_adj_create(const cnv_params* p, int checkpoint_every, double rtol_forward, double rtol_adjoint,
+                   cnv_adjoint** out) {
+    *out = nullptr;
+    if (!p || checkpoint_every < 1 || !(rtol_forward > 0) || !(rtol_adjoint > 0))
+        return afail(nullptr, CNV_ERR_ARG, "adjoint: need params, checkpoint_every >= 1 and positive tolerances");
+    if (p->precision != CNV_PREC_FP64 || p->poisson_type != 2 || p->n < kTW)
+        return afail(nullptr, CNV_ERR_ARG, "adjoint: FP64 SOR on n >= %d only", kTW);
+    cnv_adjoint* a = new cnv_adjoint();
+    a->K = checkpoint_every;
+    a->rtol_fwd = rtol_forward;
+    a->rtol_adj = rtol_adjoint;
+    auto setup = [&]() -> int {
+        cnv_params q = *p;
+        q.diagnostics = 0;
+        const int rc = cnv_create(&q, &a->c);
+        if (rc) return afail(a, rc, "%s", cnv_last_error(nullptr));
+        a->cells = a->c->cells;
+        const size_t bytes = sizeof(double) * a->cells;
+        AHIP(a, hipMalloc(&a->d_seg, bytes * 3 * a->K));
+        double** bufs[] = {&a->lw, &a->lu, &a->lv, &a->lwt, &a->lub, &a->lvb, &a->wt, &a->src};
+        for (double** b : bufs) AHIP(a, hipMalloc(b, bytes));
+        AHIP(a, hipMalloc(&a->dU, sizeof(double)));
+        return build_transposed(a);
+    };
+    const int rc = setup();
+    if (rc) {
+        char keep[256];
+        snprintf(keep, sizeof(keep), "%s", a->err);
+        cnv_adj_destroy(a);
+        snprintf(g_aerr, sizeof(g_aerr), "%s", keep);
+        return rc;
+    }
+    *out = a;
+    return CNV_OK;
+}
+
+int cnv_adj_forward(cnv_adjoint* a, const double* w0, double U, int nsteps, double* energy) {
+    if (!a || !w0 || nsteps < 1) return afail(a, CNV_ERR_ARG, "adjoint forward: bad arguments");
+    cnv_ctx* c = a->c;
+    AHIP(a, hipSetDevice(c->p.device));
+    const int n = c->n;
+    const size_t bytes = sizeof(double) * a->cells;
+    const int nck = (nsteps + a->K - 1) / a->K;
+    if (nck > a->nckpt_cap) {
+        if (a->d_ckpt) AHIP(a, hipFree(a->d_ckpt));
+        a->d_ckpt = nullptr;
+        AHIP(a, hipMalloc(&a->d_ckpt, bytes * 3 * nck));
+        a->nckpt_cap = nck;
+    }
+    // initial state: w0; u = ui, v = vi inside, 0 on the edges (src/main.c:178-221); psi = 0
+    std::vector<double> u0(a->cells, 0.0), v0(a->cells, 0.0);
+    for (int i = 1; i < n - 1; ++i)
+        for (int j = 1; j < n - 1; ++j) {
+            u0[(size_t)i * n + j] = c->p.ui;
+            v0[(size_t)i * n + j] = c->p.vi;
+        }
+    AHIP(a, hipStreamSynchronize(c->stream));
+    AHIP(a, hipMemcpy(c->w[c->wcur], w0, bytes, hipMemcpyHostToDevice));
+    AHIP(a, hipMemcpy(c->u, u0.data(), bytes, hipMemcpyHostToDevice));
+    AHIP(a, hipMemcpy(c->v, v0.data(), bytes, hipMemcpyHostToDevice));
+    c->bc.u4 = U;
+    a->U = U;
+    a->T = nsteps;
+    a->sweeps_fwd = 0;
+    for (int t = 0; t < nsteps; ++t) {
+        if (t % a->K == 0) {
+            int rc = save_state(a, a->d_ckpt, t / a->K);
+            if (rc) return rc;
+        }
+        int rc = fwd_step(a);
+        if (rc) return rc;
+    }
+    // E = 1/2 dx dy sum (u^2 + v^2), summed on the host in row-major order
+    AHIP(a, hipMemcpyAsync(u0.data(), c->u, bytes, hipMemcpyDeviceToHost, c->stream));
+    AHIP(a, hipMemcpyAsync(v0.data(), c->v, bytes, hipMemcpyDeviceToHost, c->stream));
+    AHIP(a, hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (size_t q = 0; q < a->cells; ++q) s += u0[q] * u0[q] + v0[q] * v0[q];
+    if (energy) *energy = 0.5 * c->p.dx * c->p.dy * s;
+    return CNV_OK;
+}
+
+int cnv_adj_backward(cnv_adjoint* a, double* dE_dw0, double* dE_dU) {
+    if (!a || a->T < 1) return afail(a, CNV_ERR_ARG, "adjoint backward: run cnv_adj_forward first");
+    cnv_ctx* c = a->c;
+    AHIP(a, hipSetDevice(c->p.device));
+    const int n = c->n, ld = c->ld;
+    const dim3 g = cell_grid(n), blk(kBlock);
+    // seed from the final state (the context holds it after the forward run)
+    hipLaunchKernelGGL(adj_seed_kernel, g, blk, 0, c->stream, c->u, c->v, a->lw, a->lu, a->lv, n, ld,
+                       c->p.dx * c->p.dy);
+    ALAUNCH(a);
+    hipLaunchKernelGGL(adj_zero_kernel, dim3(1), dim3(kBlock), 0, c->stream, a->dU, 1LL);
+    ALAUNCH(a);
+    a->sweeps_adj = 0;
+    const int nseg = (a->T + a->K - 1) / a->K;
+    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
+        const int t0 = sgi * a->K, t1 = std::min(a->T, t0 + a->K);
+        int rc = load_state(a, a->d_ckpt, sgi);
+        if (rc) return rc;
+        for (int t = t0; t < t1; ++t) {  // recompute the segment, storing each step's input state
+            rc = save_state(a, a->d_seg, t - t0);
+            if (!rc) rc = fwd_step(a);
+            if (rc) return rc;
+        }
+        for (int t = t1 - 1; t >= t0; --t) {
+            rc = rev_step(a, t - t0);
+            if (rc) return rc;
+        }
+    }
+    const size_t bytes = sizeof(double) * a->cells;
+    if (dE_dw0) AHIP(a, hipMemcpyAsync(dE_dw0, a->lw, bytes, hipMemcpyDeviceToHost, c->stream));
+    double dU = 0.0;
+    AHIP(a, hipMemcpyAsync(&dU, a->dU, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    AHIP(a, hipStreamSynchronize(c->stream));
+    if (dE_dU) *dE_dU = dU;
+    return CNV_OK;
+}
+
+int cnv_adj_sweeps(const cnv_adjoint* a, long long* forward, long long* adjoint) {
+    if (!a) return afail(nullptr, CNV_ERR_ARG, "null adjoint");
+    if (forward) *forward = a->sweeps_fwd;
+    if (adjoint) *adjoint = a->sweeps_adj;
+    return CNV_OK;
+}

@@ -147,6 +147,30 @@ int cnv_batch_get_state(cnv_batch *b, int i, double *w, double *psi, double *u, 
 int cnv_batch_autotune(cnv_batch *b, int launches, int *sweeps_per_launch, int *tile_waves,
                        int *wave_rows, double *ms_per_sweep);
 
+/* ---- adjoint (reverse-mode gradient) -----------------------------------------------------------
+ * Objective: E = 1/2 dx dy * sum over all points of (u^2 + v^2) after nsteps FP64 steps of the
+ * cavity described by p, started from the vorticity field w0 (u = ui, v = vi inside, 0 on the
+ * walls) with lid velocity U (u4).  Gradients dE/dw0 (n*n, exactly 0 on the walls: step 0
+ * overwrites the wall vorticity before reading it) and dE/dU by hand-written adjoint kernels of
+ * the whole step (transposed advection/diffusion stencils, the self-adjoint Poisson solve, the
+ * adjoints of the velocity and wall-vorticity boundary conditions), with the forward state
+ * checkpointed every checkpoint_every steps in device memory and recomputed per segment.
+ * Every Poisson solve of the adjoint problem (forward, recompute, adjoint) stops when its L1
+ * update falls below rtol times the L1 norm of the iterate (rtol_forward / rtol_adjoint), so
+ * that the gradient is that of a fully converged trajectory.  FP64, SOR (poisson_type 2), n >= 10.
+ * No counterpart in the reference. */
+typedef struct cnv_adjoint cnv_adjoint;
+int cnv_adj_create(const cnv_params *p, int checkpoint_every, double rtol_forward, double rtol_adjoint,
+                   cnv_adjoint **out);
+void cnv_adj_destroy(cnv_adjoint *a);
+const char *cnv_adj_last_error(const cnv_adjoint *a);   /* a may be NULL */
+/* Forward run from (w0, U) for nsteps; stores the checkpoints; returns E. */
+int cnv_adj_forward(cnv_adjoint *a, const double *w0, double U, int nsteps, double *energy);
+/* Gradient of the last forward run's E: dE_dw0 (host, n*n) and dE_dU (either may be NULL). */
+int cnv_adj_backward(cnv_adjoint *a, double *dE_dw0, double *dE_dU);
+/* SOR sweeps of the last forward run and of the last backward run's adjoint solves. */
+int cnv_adj_sweeps(const cnv_adjoint *a, long long *forward, long long *adjoint);
+
 #ifdef __cplusplus
 }
 #endif

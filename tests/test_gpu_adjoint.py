@@ -1,0 +1,98 @@
+"""GPU tests of the adjoint (cnv_adj_*): the hand-written adjoint kernels' gradient of the final
+kinetic energy against the FP64 CPU adjoint of the test harness (adjoint_ref.py, itself pinned by
+central differences on the CPU), against central finite differences (eps = 1e-6) of the GPU
+forward run, the wall invariant dE/dw0 = 0 (step 0 overwrites the wall vorticity before reading
+it, src/main.c:311-320), independence of the checkpoint interval, and the gradient seen through a
+torch.autograd.Function wrapper (test harness only)."""
+import numpy as np
+import pytest
+
+import cnv
+from adjoint_ref import CavityAdjoint
+
+pytestmark = pytest.mark.gpu
+
+
+def make(n, Re, dt, K):
+    p = cnv.default_params(n, Re=Re, dt=dt, beta=cnv.reference_beta(n), itmax=10_000_000, diagnostics=0)
+    return cnv.Adjoint(p, checkpoint_every=K, rtol_forward=1e-14, rtol_adjoint=1e-14)
+
+
+def walls(n):
+    E = np.zeros((n, n), dtype=bool)
+    E[0, :] = E[-1, :] = E[:, 0] = E[:, -1] = True
+    return E
+
+
+def test_gpu_adjoint_matches_cpu_adjoint():
+    n, steps, U, Re, dt = 24, 6, 1.1, 400.0, 4e-3
+    w0 = cnv.gaussian_vorticity(n, count=8, seed=2)
+    print("adjoint n=%d steps=%d" % (n, steps), flush=True)
+    with make(n, Re, dt, K=2) as adj:
+        E = adj.forward(w0, U, steps)
+        g, dU = adj.backward()
+        fs, bs = adj.sweeps()
+    E_ref, g_ref, dU_ref = CavityAdjoint(n, Re, dt).gradient(w0, U, steps)
+    print("E", E, E_ref, "dU", dU, dU_ref, "sweeps", fs, bs, flush=True)
+    assert abs(E - E_ref) <= 1e-11 * abs(E_ref)
+    assert np.max(np.abs(g - g_ref)) <= 1e-8 * np.max(np.abs(g_ref))
+    assert abs(dU - dU_ref) <= 1e-8 * abs(dU_ref)
+    assert np.all(g[walls(n)] == 0.0)
+    assert fs > 0 and bs > 0
+
+
+def test_gpu_adjoint_central_differences():
+    n, steps, U, Re, dt = 32, 8, 0.9, 400.0, 3e-3
+    rng = np.random.default_rng(7)
+    w0 = cnv.gaussian_vorticity(n, count=12, seed=1)
+    with make(n, Re, dt, K=3) as adj:
+        adj.forward(w0, U, steps)
+        g, dU = adj.backward()
+        eps = 1e-6
+        for trial in range(2):
+            d = rng.standard_normal((n, n))
+            fd = (adj.forward(w0 + eps * d, U, steps) - adj.forward(w0 - eps * d, U, steps)) / (2 * eps)
+            print("direction", trial, float(np.sum(g * d)), fd, flush=True)
+            assert abs(np.sum(g * d) - fd) <= 1e-5 * abs(fd)
+        fdU = (adj.forward(w0, U + eps, steps) - adj.forward(w0, U - eps, steps)) / (2 * eps)
+        print("lid", dU, fdU, flush=True)
+        assert abs(dU - fdU) <= 1e-5 * abs(fdU)
+
+
+def test_gpu_adjoint_checkpoint_interval_does_not_change_the_gradient():
+    n, steps, U, Re, dt = 20, 7, 1.0, 250.0, 4e-3
+    w0 = cnv.gaussian_vorticity(n, count=6, seed=4)
+    out = []
+    for K in (1, 3, 7, 100):
+        with make(n, Re, dt, K) as adj:
+            E = adj.forward(w0, U, steps)
+            g, dU = adj.backward()
+        out.append((E, g, dU))
+    for E, g, dU in out[1:]:
+        assert E == out[0][0] and dU == out[0][2] and np.array_equal(g, out[0][1])
+
+
+def test_gpu_adjoint_through_torch_autograd():
+    torch = pytest.importorskip("torch")
+    n, steps, Re, dt = 16, 4, 300.0, 4e-3
+    adj = make(n, Re, dt, K=2)
+
+    class CavityEnergy(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, w0, U):
+            return torch.tensor(adj.forward(w0.detach().numpy(), float(U), steps), dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gE):
+            g, dU = adj.backward()
+            return gE * torch.from_numpy(g), gE * torch.tensor(dU, dtype=torch.float64)
+
+    w0 = torch.from_numpy(cnv.gaussian_vorticity(n, count=5, seed=9)).requires_grad_(True)
+    U = torch.tensor(1.2, dtype=torch.float64, requires_grad=True)
+    E = CavityEnergy.apply(w0, U)
+    E.backward()
+    E_ref, g_ref, dU_ref = CavityAdjoint(n, Re, dt).gradient(w0.detach().numpy(), 1.2, steps)
+    assert abs(float(E.detach()) - E_ref) <= 1e-11 * abs(E_ref)
+    assert np.max(np.abs(w0.grad.numpy() - g_ref)) <= 1e-8 * np.max(np.abs(g_ref))
+    assert abs(float(U.grad) - dU_ref) <= 1e-8 * abs(dU_ref)
+    adj.close()
