@@ -129,6 +129,42 @@ def batched_leg(cnv, steps, precision, tag):
     }
 
 
+def adjoint_leg(cnv, steps, K):
+    """The adjoint workload: 256^2 cavity, Re 400, lid U = 1, dt = 20*2^-16, initial vorticity a sum
+    of 32 Gaussians (seed 1); gradient of the final kinetic energy w.r.t. the initial vorticity and
+    the lid velocity, checkpoints every K steps; checked by central differences (eps = 1e-6) along
+    one random direction and in U."""
+    n, Re, U, dt = 256, 400.0, 1.0, 20 * 2.0 ** -16
+    p = cnv.default_params(n, Re=Re, dt=dt, beta=cnv.reference_beta(n), itmax=50_000_000, diagnostics=0)
+    w0 = cnv.gaussian_vorticity(n, count=32, seed=1)
+    with cnv.Adjoint(p, checkpoint_every=K, rtol_forward=1e-14, rtol_adjoint=1e-14) as adj:
+        t0 = time.perf_counter()
+        E = adj.forward(w0, U, steps)
+        t_fwd = time.perf_counter() - t0
+        fs, _ = adj.sweeps()
+        t0 = time.perf_counter()
+        g, dU = adj.backward()
+        t_bwd = time.perf_counter() - t0
+        _, bs = adj.sweeps()
+        print("bench: adjoint forward %.2f s, backward %.2f s" % (t_fwd, t_bwd), file=sys.stderr, flush=True)
+        eps = 1e-6
+        d = np.random.default_rng(11).standard_normal((n, n))
+        fd = (adj.forward(w0 + eps * d, U, steps) - adj.forward(w0 - eps * d, U, steps)) / (2 * eps)
+        fdU = (adj.forward(w0, U + eps, steps) - adj.forward(w0, U - eps, steps)) / (2 * eps)
+    walls = np.concatenate([g[0, :], g[-1, :], g[:, 0], g[:, -1]])
+    return {
+        "workload": "cavity 256x256 Re400 FP64, 2000 steps, w0 = 32 Gaussians (seed 1), E = final kinetic "
+                    "energy, checkpoints every %d steps, Poisson solves to relative L1 update 1e-14" % K,
+        "steps": steps, "energy": E, "dE_dU": dU,
+        "forward_s": t_fwd, "backward_s": t_bwd,
+        "gradient_over_forward": t_bwd / t_fwd,
+        "sweeps_forward": fs, "sweeps_backward_adjoint_solves": bs,
+        "fd_rel_err_direction": abs(float(np.sum(g * d)) - fd) / abs(fd),
+        "fd_rel_err_lid": abs(dU - fdU) / abs(fdU),
+        "wall_gradient_max_abs": float(np.max(np.abs(walls))),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +178,8 @@ def main():
     ap.add_argument("--sor-launches", type=int, default=400)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batched leg")
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 256^2 adjoint leg")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--profile-sor", type=int, default=0, metavar="LAUNCHES",
                     help="profiling mode: one step, then LAUNCHES timed SOR launches and one 256 MiB "
                          "calibration copy (for counter scaling); prints no JSON")
@@ -223,6 +261,11 @@ def main():
         batched = batched_leg(cnv, args.batch_steps, cnv.PREC_FP32_DC, "fp32-refined")
         batched["fp64"] = batched_leg(cnv, args.batch_steps, cnv.PREC_FP64, "fp64")
 
+    # ---- adjoint workload ---------------------------------------------------------------------
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(cnv, args.adjoint_steps, 100)
+
     # ---- CPU baseline -----------------------------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -258,6 +301,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32_refined": dc,
         "batched": batched,
+        "adjoint": adjoint,
         "device": dev,
     }
     print(json.dumps(line))
