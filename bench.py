@@ -37,9 +37,11 @@ BYTES_PER_CELL_SWEEP_FP32 = 12
 
 
 def workload_params(cnv, precision, args):
+    # FP64: each step replays captured HIP graphs unless --eager
     return cnv.default_params(N, Re=RE, dt=DT, order=6, beta=OMEGA, tol=TOL, itmax=50_000_000,
                               precision=precision, sweeps_per_launch=args.spl, tile_waves=args.tile_waves,
-                              wave_rows=args.wave_rows, diagnostics=1)
+                              wave_rows=args.wave_rows, diagnostics=1,
+                              use_graph=0 if args.eager else int(precision == cnv.PREC_FP64))
 
 
 def timed_steps(cav, k, tag):
@@ -174,6 +176,7 @@ def main():
     ap.add_argument("--tile-waves", type=int, default=0)
     ap.add_argument("--wave-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--eager", action="store_true", help="FP64 steps as eager launches instead of graph replays")
     ap.add_argument("--no-dc", action="store_true", help="skip the FP32-refined leg")
     ap.add_argument("--sor-launches", type=int, default=400)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batched leg")
@@ -293,6 +296,7 @@ def main():
         "data": "synthetic (deterministic lid-driven cavity from rest)",
         "config": {"workload": "cavity 1024x1024 Re1000 FP64 red-black SOR omega=1.95 tol=1e-10, order 6",
                    "n": N, "Re": RE, "dt": DT, "sor_omega": OMEGA, "tol": TOL, "order": 6,
+                   "step": "eager launches" if args.eager else "captured HIP graphs",
                    "sweeps_per_launch": spl, "sweeps_per_step": sweeps,
                    "algorithmic_GBps_step_loop": alg_bytes / T / 1e9},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",

@@ -25,7 +25,7 @@ class Params(C.Structure):
         ("v1", C.c_double), ("v2", C.c_double), ("v3", C.c_double), ("v4", C.c_double),
         ("inner_rtol", C.c_double), ("max_outer", C.c_int),
         ("sweeps_per_launch", C.c_int), ("tile_waves", C.c_int), ("wave_rows", C.c_int),
-        ("launches_per_poll", C.c_int), ("diagnostics", C.c_int),
+        ("launches_per_poll", C.c_int), ("diagnostics", C.c_int), ("use_graph", C.c_int),
     ]
 
 

@@ -17,6 +17,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -202,9 +203,18 @@ static int initial_state(cnv_ctx* c) {
     return cnv_set_state(c, a.data(), a.data(), uu.data(), vv.data());
 }
 
+static void drop_graphs(cnv_ctx* c) {
+    hipGraphExec_t* gs[] = {&c->g_pre[0], &c->g_pre[1], &c->g_chunk[0], &c->g_chunk[1], &c->g_post};
+    for (hipGraphExec_t* g : gs) {
+        if (*g) (void)hipGraphExecDestroy(*g);
+        *g = nullptr;
+    }
+}
+
 void cnv_destroy(cnv_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     void* dev[] = {c->w[0], c->w[1], c->psi[0], c->psi[1], c->u, c->v, c->d_start, c->d_tab,
                    c->d_ftab, c->ctl, c->diag, c->partial, c->e32[0], c->e32[1], c->r32};
     for (void* p : dev)
@@ -332,7 +342,132 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) 
     return CNV_OK;
 }
 
+// ---- captured step (use_graph, FP64) -------------------------------------------------------
+// Launch index L only matters through its parity and every solve's state lives in the device
+// slots, so one captured chunk of M (even) SOR launches with L = 0..M-1 is replayed until the
+// solve ends: each replay continues the previous one's slot parity, and the final slot is 1.
+static int capture(cnv_ctx* c, hipGraphExec_t* out, int (*body)(cnv_ctx*, int), int arg) {
+    hipGraph_t g = nullptr;
+    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = body(c, arg);
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    return CNV_OK;
+}
+
+// wall vorticity and transport of w[wc] into w[wc^1], then a fresh Poisson solve
+static int body_pre(cnv_ctx* c, int wc) {
+    const int n = c->n, ld = c->ld;
+    LAUNCH(c, wall_vorticity_launch(c->w[wc], c->u, c->v, n, ld, c->st, c->bc, c->stream));
+    LAUNCH(c, transport_fp64_launch(c->w[wc], c->w[wc ^ 1], c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re, c->p.dt,
+                                    c->stream));
+    LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, n, ld, c->p.tol, c->p.itmax, c->stream));
+    return CNV_OK;
+}
+// M SOR launches, L = 0..M-1, on the source w[src] (f = -w)
+static int body_chunk(cnv_ctx* c, int src) {
+    const SorArgs<double> a = sor_args<double>(c, c->w[src], c->psi[0], c->psi[1], 1);
+    const bool gs = c->p.poisson_type == 1;
+    for (int l = 0; l < c->launches_per_poll; ++l)
+        if (launch_sor(c, a, l, gs)) return cnv_fail(c, CNV_ERR_HIP, "SOR launch failed while capturing");
+    return CNV_OK;
+}
+// velocities from the final iterate (slot 1 after whole chunks), continuity diagnostic
+static int body_post(cnv_ctx* c, int) {
+    LAUNCH(c, velocity_launch(c->psi[0], c->psi[1], c->ctl + 1, c->u, c->v, c->n, c->ld, c->st, c->stream));
+    if (c->p.diagnostics) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, c->n, c->ld, c->st, c->stream));
+    return CNV_OK;
+}
+
+static int build_graphs(cnv_ctx* c) {
+    if (c->launches_per_poll & 1) ++c->launches_per_poll;  // whole chunks end on slot 1
+    for (int wc = 0; wc < 2; ++wc) {
+        int rc = capture(c, &c->g_pre[wc], body_pre, wc);
+        if (!rc) rc = capture(c, &c->g_chunk[wc], body_chunk, wc);
+        if (rc) return rc;
+    }
+    return capture(c, &c->g_post, body_post, 0);
+}
+
+// replays the chunk graph of source plane src until a polled slot is terminal
+static int run_sor_graph(cnv_ctx* c, int src, SorCtl* final_ctl) {
+    const int M = c->launches_per_poll;
+    const int S = c->shape.s;
+    const long long cap = (long long)c->p.itmax + 4LL * M + 8;
+    int first = 1;  // chunks queued before the first poll
+    if (c->k_prev > 0) first = std::max(1, (c->k_prev / S - M + M - 1) / M);
+    long long queued = 0;
+    int slot = 0;
+    int pending[kSlots];
+    int npend = 0;
+    c->fin_slot = 1;
+    while (true) {
+        const int chunks = queued == 0 ? first : 1;
+        for (int k = 0; k < chunks; ++k) HIPC(c, hipGraphLaunch(c->g_chunk[src], c->stream));
+        queued += (long long)chunks * M;
+        HIPC(c, hipMemcpyAsync(&c->h_ctl[slot], c->ctl + 1, sizeof(SorCtl), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipEventRecord(c->ev[slot], c->stream));
+        pending[npend++] = slot;
+        slot = (slot + 1) % kSlots;
+        if (npend >= 2) {
+            const int s0 = pending[0];
+            for (int q = 1; q < npend; ++q) pending[q - 1] = pending[q];
+            --npend;
+            HIPC(c, hipEventSynchronize(c->ev[s0]));
+            const SorCtl snap = c->h_ctl[s0];
+            if (sor_done(snap.status)) {
+                *final_ctl = snap;
+                return CNV_OK;
+            }
+        }
+        if (queued > cap + (long long)first * M)
+            return cnv_fail(c, CNV_ERR_HIP, "SOR state did not terminate after %lld launches", queued);
+    }
+}
+
+static int one_step(cnv_ctx* c, cnv_step_stats* stats);
+
+static int one_step_graph(cnv_ctx* c, cnv_step_stats* stats) {
+    if (!c->g_post) {
+        const int rc = build_graphs(c);
+        if (rc) {
+            drop_graphs(c);
+            return rc;
+        }
+    }
+    HIPC(c, hipGraphLaunch(c->g_pre[c->wcur], c->stream));
+    c->wcur ^= 1;
+    SorCtl fin{};
+    int rc = run_sor_graph(c, c->wcur, &fin);
+    if (rc) return rc;
+    c->k_prev = fin.k;
+    HIPC(c, hipGraphLaunch(c->g_post, c->stream));
+    cnv_step_stats s{};
+    s.status = fin.status == SOR_CONVERGED ? CNV_OK : CNV_NOT_CONVERGED;
+    s.sweeps = fin.k;
+    s.k = fin.k - 1;
+    s.e = fin.e_last;
+    if (c->p.diagnostics) {
+        HIPC(c, hipMemcpyAsync(c->h_diag, c->diag, sizeof(Diag), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        s.cont_max = dkey_inv(c->h_diag->kmax);
+        s.cont_min = dkey_inv(c->h_diag->kmin);
+    }
+    if (stats) *stats = s;
+    if (s.status != CNV_OK)
+        return cnv_fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
+    return CNV_OK;
+}
+
 static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
+    if (c->p.use_graph && c->p.precision == CNV_PREC_FP64) return one_step_graph(c, stats);
     const int n = c->n, ld = c->ld;
     const double rRe = 1. / c->p.Re;  // src/fluiddyn.c:98 "(1. / Re)"
     double* w_in = c->w[c->wcur];
@@ -540,5 +675,7 @@ int cnv_set_poisson_params(cnv_ctx* c, double dx, double dy, double beta, double
     c->p.tol = tol;
     c->p.itmax = itmax;
     c->p.poisson_type = poisson_type;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);  // the captured launches hold the old coefficients
     return CNV_OK;
 }

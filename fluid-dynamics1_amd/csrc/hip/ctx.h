@@ -46,6 +46,11 @@ struct cnv_ctx {
     int launches_per_poll = 32;
     int k_prev = 0;       // sweeps of the previous solve (queue-ahead prediction)
     cnv::WallBC bc{};
+    // captured step (use_graph, FP64): pre-solve launches per w parity, a chunk of
+    // launches_per_poll SOR launches per source plane, post-solve launches
+    hipGraphExec_t g_pre[2] = {nullptr, nullptr};
+    hipGraphExec_t g_chunk[2] = {nullptr, nullptr};
+    hipGraphExec_t g_post = nullptr;
     char err[256] = "";
 };
 

@@ -47,7 +47,7 @@ typedef struct {
     double u1, u2, u3, u4; /* wall velocities: column 0, column n-1, row 0, row n-1          */
     double v1, v2, v3, v4;
     /* FP32_DC only */
-    double inner_rtol;     /* inner FP32 solve: stop at this fraction of its first update    */
+    double inner_rtol;     /* inner FP32 solve: fold once its L1 update < inner_rtol * L1 norm of the inner iterate */
     int max_outer;         /* defect-correction iterations allowed per solve                */
     /* tuning (0 = automatic) */
     int sweeps_per_launch; /* SOR sweeps fused per kernel launch (1, 2, 4, 6 or 8)           */
@@ -55,6 +55,9 @@ typedef struct {
     int wave_rows;         /* grid rows held in registers per wave (compiled shapes only)    */
     int launches_per_poll; /* SOR launches queued between host polls of the convergence flag */
     int diagnostics;       /* 1 = continuity max/min every step (src/main.c:387-408)        */
+    int use_graph;         /* FP64: 1 = each step replays captured HIP graphs (the pre-solve
+                              launches, chunks of launches_per_poll SOR launches, the post-solve
+                              launches); results identical to the eager launches              */
 } cnv_params;
 
 typedef struct {
