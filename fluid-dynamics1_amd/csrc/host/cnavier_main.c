@@ -132,6 +132,7 @@ int main(int argc, char *argv[])
     p.inner_rtol = cx.inner_rtol;
     p.sweeps_per_launch = cx.sweeps_per_launch;
     p.diagnostics = cx.diagnostics;
+    p.use_graph = cx.use_graph;
     cnv_ctx *ctx = NULL;
     int rc = cnv_create(&p, &ctx);
     if (rc != CNV_OK) {

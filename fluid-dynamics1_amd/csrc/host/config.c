@@ -81,6 +81,7 @@ static int ext_key(ConfigExt *x, const char *k, const char *v)
     if (strcmp(k, "sweeps_per_launch") == 0) { x->sweeps_per_launch = atoi(v); return 1; }
     if (strcmp(k, "diagnostics") == 0) { x->diagnostics = atoi(v); return 1; }
     if (strcmp(k, "device") == 0) { x->device = atoi(v); return 1; }
+    if (strcmp(k, "use_graph") == 0) { x->use_graph = atoi(v); return 1; }
     return 0;
 }
 
@@ -166,7 +167,7 @@ void print_usage(const char *program_name)
     printf("  output_folder  results go to ./output/<output_folder>/ (default ./output/)\n");
     printf("\nEvery step runs on the GPU.  Keys added to the reference's set:\n");
     printf("  sor_omega, nsteps, precision (fp64 | fp32dc), inner_rtol, sweeps_per_launch,\n");
-    printf("  diagnostics, device\n");
+    printf("  diagnostics, device, use_graph\n");
     printf("\nFor the list of parameters, run with --help-config\n");
 }
 

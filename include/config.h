@@ -40,6 +40,7 @@ typedef struct {
     int sweeps_per_launch;  /* sweeps_per_launch: SOR sweeps per kernel launch (0 = default)   */
     int diagnostics;        /* diagnostics: 1 = continuity max/min every step                  */
     int device;             /* device: HIP device ordinal                                        */
+    int use_graph;          /* use_graph: 1 = replay each FP64 step from captured HIP graphs    */
     int has_sor_omega;
 } ConfigExt;
 

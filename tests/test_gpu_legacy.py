@@ -75,6 +75,9 @@ def _run_driver(tmp_path, cfg_text, name):
     ("testRunOMPHIGHRES", "Re = 5000.0\nnx = 128\nny = 128\ndt = 0.001\ntf = 10.0\nmax_co = 0.5\n"
                           "order = 6\npoisson_max_it = 15000\npoisson_tol = 5E-4\npoisson_type = 2\n"
                           "output_interval = 100\nnsteps = 11\n"),
+    ("testRunOMPHIGHRES", "Re = 5000.0\nnx = 128\nny = 128\ndt = 0.001\ntf = 10.0\nmax_co = 0.5\n"
+                          "order = 6\npoisson_max_it = 15000\npoisson_tol = 5E-4\npoisson_type = 2\n"
+                          "output_interval = 100\nnsteps = 11\nuse_graph = 1\n"),
 ])
 def test_driver_reproduces_reference_openmp_log(tmp_path, run, cfg):
     g = np.load(os.path.join(GOLD, "log_kat.npz"))
