@@ -34,6 +34,7 @@ TOL = 1e-10
 HBM_PEAK = 8.0e12             # MI355X HBM3E, bytes/s
 BYTES_PER_CELL_SWEEP = 24     # read psi, read omega, write psi (FP64)
 BYTES_PER_CELL_SWEEP_FP32 = 12
+C_CELLS = N * N
 
 
 def workload_params(cnv, precision, args):
@@ -167,6 +168,47 @@ def adjoint_leg(cnv, steps, K):
     }
 
 
+def dc_leg(cnv, args, state64, ms64, spl, inner_beta, tag, time_sor=True):
+    """The FP32 + FP64 defect-correction form of the same steps, compared with the FP64 run."""
+    p = workload_params(cnv, cnv.PREC_FP32_DC, args)
+    p.inner_beta = inner_beta
+    cd = cnv.Cavity(p)
+    if args.warmup:
+        timed_steps(cd, args.warmup, tag + " warmup")
+    st32, ms32 = timed_steps(cd, args.steps, tag)
+    s32 = cd.get_state()
+    sor32_ms = cd.time_sor(args.sor_launches) if time_sor else None
+    cd.close()
+    mid = N // 2
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    def rel2(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    out = {
+        "inner_beta": inner_beta if inner_beta > 0 else OMEGA,
+        "value": C_CELLS * args.steps / (ms32 / 1e3) / 1e6,
+        "ms_per_step": ms32 / args.steps,
+        "speedup_vs_fp64": ms64 / ms32,
+        "inner_sweeps_per_step": [s.sweeps for s in st32],
+        "outer_per_step": [s.outer for s in st32],
+        "err_linf_rel": {"psi": rel(s32["psi"], state64["psi"]), "omega": rel(s32["w"], state64["w"]),
+                         "u_centerline": rel(s32["u"][:, mid], state64["u"][:, mid]),
+                         "v_centerline": rel(s32["v"][mid, :], state64["v"][mid, :])},
+        "err_l2_rel": {"psi": rel2(s32["psi"], state64["psi"]), "omega": rel2(s32["w"], state64["w"]),
+                       "u_centerline": rel2(s32["u"][:, mid], state64["u"][:, mid]),
+                       "v_centerline": rel2(s32["v"][mid, :], state64["v"][mid, :])},
+    }
+    if sor32_ms is not None:
+        out["sor_fp32_launch_ms"] = sor32_ms
+        out["sor_fp32_achieved_GBps"] = spl * BYTES_PER_CELL_SWEEP_FP32 * C_CELLS / (sor32_ms / 1e3) / 1e9
+    print("bench: %s %.3f ms/step, inner sweeps %s" % (tag, ms32 / args.steps, out["inner_sweeps_per_step"]),
+          file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,35 +270,11 @@ def main():
     # ---- FP32 + FP64 defect correction ------------------------------------------------------
     dc = None
     if not args.no_dc:
-        cd = cnv.Cavity(workload_params(cnv, cnv.PREC_FP32_DC, args))
-        if args.warmup:
-            timed_steps(cd, args.warmup, "dc warmup")
-        st32, ms32 = timed_steps(cd, args.steps, "dc")
-        s32 = cd.get_state()
-        sor32_ms = cd.time_sor(args.sor_launches)
-        cd.close()
-        mid = N // 2
-        def rel(a, b):
-            return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
-        def rel2(a, b):
-            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
-        dc = {
-            "value": C * args.steps / (ms32 / 1e3) / 1e6,
-            "ms_per_step": ms32 / args.steps,
-            "speedup_vs_fp64": ms / ms32,
-            "inner_sweeps_per_step": [s.sweeps for s in st32],
-            "outer_per_step": [s.outer for s in st32],
-            "err_linf_rel": {"psi": rel(s32["psi"], state64["psi"]), "omega": rel(s32["w"], state64["w"]),
-                             "u_centerline": rel(s32["u"][:, mid], state64["u"][:, mid]),
-                             "v_centerline": rel(s32["v"][mid, :], state64["v"][mid, :])},
-            "err_l2_rel": {"psi": rel2(s32["psi"], state64["psi"]), "omega": rel2(s32["w"], state64["w"]),
-                           "u_centerline": rel2(s32["u"][:, mid], state64["u"][:, mid]),
-                           "v_centerline": rel2(s32["v"][mid, :], state64["v"][mid, :])},
-            "sor_fp32_launch_ms": sor32_ms,
-            "sor_fp32_achieved_GBps": spl * BYTES_PER_CELL_SWEEP_FP32 * C / (sor32_ms / 1e3) / 1e9,
-        }
-        print("bench: dc %.3f ms/step, inner sweeps %s" % (ms32 / args.steps, dc["inner_sweeps_per_step"]),
-              file=sys.stderr, flush=True)
+        dc = dc_leg(cnv, args, state64, ms, spl, 0.0, "dc")
+        # the same form with the inner FP32 solves relaxed by the reference's own rule
+        # (src/main.c:134) instead of omega = 1.95: an algorithmic choice open to the inner solver
+        dc["reference_inner_beta"] = dc_leg(cnv, args, state64, ms, spl, cnv.reference_beta(N), "dc-refbeta",
+                                            time_sor=False)
 
     # ---- batched 64-cavity workload -----------------------------------------------------------
     batched = None

@@ -97,8 +97,10 @@ inline cnv::SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int nega
     a.c2 = (T)(dx * dx);
     a.c3 = (T)(dx * dx * dy * dy);
     a.B = (T)(2 * (dx * dx + dy * dy));
-    a.beta = (T)c->p.beta;
-    a.omb = (T)(1 - c->p.beta);
+    // FP32 planes (the defect-correction inner solve) may relax with their own factor
+    const double beta = (sizeof(T) == 4 && c->p.inner_beta > 0) ? c->p.inner_beta : c->p.beta;
+    a.beta = (T)beta;
+    a.omb = (T)(1 - beta);
     a.rB = (T)(1.0 / (2 * (dx * dx + dy * dy)));
     int e2 = 0;
     const double Bd = 2 * (dx * dx + dy * dy);

@@ -58,6 +58,7 @@ typedef struct {
     int use_graph;         /* FP64: 1 = each step replays captured HIP graphs (the pre-solve
                               launches, chunks of launches_per_poll SOR launches, the post-solve
                               launches); results identical to the eager launches              */
+    double inner_beta;     /* FP32_DC: relaxation of the inner FP32 SOR solves (0 = beta)     */
 } cnv_params;
 
 typedef struct {
