@@ -232,7 +232,7 @@ struct cnv_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<cnv_ctx*> ctx;
-    TileShape shape{8, 8, 10};
+    TileShape shape{5, 16, 4};
     int launches_per_poll = 16;
     int wcur = 0;
     int fin_slot = 0;

@@ -51,6 +51,8 @@ double cnv_reference_beta(int nx, int ny) {
     return 0.5 * (2 / (1 + sin(pi / (nx + 1))) + 2 / (1 + sin(pi / (ny + 1))));  // src/main.c:134
 }
 
+size_t cnv_params_size(void) { return sizeof(cnv_params); }
+
 void cnv_default_params(cnv_params* p, int n) {
     memset(p, 0, sizeof(*p));
     p->n = n;
@@ -92,14 +94,16 @@ int cnv_device_name(int device, char* buf, int len) {
     return CNV_OK;
 }
 
-// Default launch shape per sweeps-per-launch (tuned on MI355X at 1024^2).
+// Default launch shape per sweeps-per-launch (tuned on MI355X at 1024^2; 0 = the default).
 static TileShape shape_for(int spl) {
     switch (spl) {
+        case 0: return TileShape{5, 16, 4};
         case 1: return TileShape{1, 4, 10};
         case 2: return TileShape{2, 4, 10};
         case 4: return TileShape{4, 8, 8};
+        case 5: return TileShape{5, 16, 4};
         case 6: return TileShape{6, 8, 8};
-        default: return TileShape{8, 8, 10};
+        default: return TileShape{spl, 8, 10};
     }
 }
 
@@ -251,7 +255,7 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
     c->ld = p->n;
     c->cells = (size_t)p->n * p->n;
     c->bc = WallBC{p->u1, p->u2, p->u3, p->u4, p->v1, p->v2, p->v3, p->v4};
-    int spl = p->sweeps_per_launch ? p->sweeps_per_launch : 8;
+    const int spl = p->sweeps_per_launch;
     c->shape = shape_for(spl);
     if (p->tile_waves > 0) c->shape.nw = p->tile_waves;
     if (p->wave_rows > 0) c->shape.rb = p->wave_rows;

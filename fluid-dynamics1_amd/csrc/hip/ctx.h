@@ -41,7 +41,7 @@ struct cnv_ctx {
     float* e32[2] = {nullptr, nullptr};
     float* r32 = nullptr;
     // launch shape
-    cnv::TileShape shape{8, 8, 10};
+    cnv::TileShape shape{5, 16, 4};
     int tiles_i = 0, tiles_j = 0;
     int launches_per_poll = 32;
     int k_prev = 0;       // sweeps of the previous solve (queue-ahead prediction)

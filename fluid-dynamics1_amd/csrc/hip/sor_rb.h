@@ -433,10 +433,14 @@ inline int sor_launch_shape(const SorArgs<T>& a, bool gs, hipStream_t st) {
     const dim3 grid(ntiles), block(NW * 64);
     // FP32 always multiplies by RN(1/B); FP64 does when 1/B is exact
     const bool xr = sizeof(T) == 4 || a.exact_rcp;
-    if (sizeof(T) == 8 && a.norm && !gs) {  // FP64 with the iterate norm (relative stopping rule)
-        if (xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, true, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, false, true>), grid, block, 0, st, a);
-    } else if (gs && xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, true, true>), grid, block, 0, st, a);
+    if constexpr (sizeof(T) == 8) {
+        if (a.norm && !gs) {  // FP64 with the iterate norm (relative stopping rule)
+            if (xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, true, true>), grid, block, 0, st, a);
+            else hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, false, true>), grid, block, 0, st, a);
+            return hipGetLastError() == hipSuccess ? 0 : 1;
+        }
+    }
+    if (gs && xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, true, true>), grid, block, 0, st, a);
     else if (gs) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, true, false>), grid, block, 0, st, a);
     else if (xr) hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, true>), grid, block, 0, st, a);
     else hipLaunchKernelGGL((sor_rb_kernel<T, S, NW, RB, false, false>), grid, block, 0, st, a);
@@ -444,15 +448,19 @@ inline int sor_launch_shape(const SorArgs<T>& a, bool gs, hipStream_t st) {
 }
 
 // The launch shapes compiled in, TileShape {s, nw, rb}: S sweeps per launch, nw waves of rb rows;
-// the tile owns (nw*rb - 4s) rows x (128 - 4s) columns.
+// the tile owns (nw*rb - 4s) rows x (128 - 4s) columns.  (5, 16, 4) is the default: at 1024^2 it
+// gives 240 workgroups of 16 waves (four per SIMD), 1.72x halo redundancy; the others serve
+// small grids, the batched autotuner and the sweep tests.
 #define CNV_SOR_SHAPES(X) \
     X(1, 4, 10)           \
     X(2, 4, 10)           \
-    X(4, 4, 12)           \
     X(4, 8, 8)            \
     X(6, 8, 8)            \
     X(8, 8, 10)           \
-    X(8, 8, 12)
+    X(4, 16, 4)           \
+    X(5, 16, 4)           \
+    X(6, 16, 4)           \
+    X(5, 12, 6)
 
 template <typename T>
 inline int sor_dispatch(const SorArgs<T>& a, TileShape sh, bool gs, hipStream_t st) {

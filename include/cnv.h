@@ -16,6 +16,8 @@
 #ifndef CNV_H
 #define CNV_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -50,7 +52,7 @@ typedef struct {
     double inner_rtol;     /* inner FP32 solve: fold once its L1 update < inner_rtol * L1 norm of the inner iterate */
     int max_outer;         /* defect-correction iterations allowed per solve                */
     /* tuning (0 = automatic) */
-    int sweeps_per_launch; /* SOR sweeps fused per kernel launch (1, 2, 4, 6 or 8)           */
+    int sweeps_per_launch; /* SOR sweeps fused per kernel launch (compiled: 1, 2, 4, 5, 6, 8; 0 = 5) */
     int tile_waves;        /* wave64s per SOR workgroup (compiled shapes only)               */
     int wave_rows;         /* grid rows held in registers per wave (compiled shapes only)    */
     int launches_per_poll; /* SOR launches queued between host polls of the convergence flag */
@@ -73,6 +75,8 @@ typedef struct {
 /* Reference defaults for an n x n cavity: load_default_config() values (src/config.c:47-87)
  * except n, with beta from src/main.c:134 and dx = 1/n. */
 void cnv_default_params(cnv_params *p, int n);
+/* sizeof(cnv_params) as compiled into the library: bindings check their layout against it. */
+size_t cnv_params_size(void);
 double cnv_reference_beta(int nx, int ny);
 
 int cnv_create(const cnv_params *p, cnv_ctx **out);

@@ -147,3 +147,10 @@ def test_batch_workload_table_is_seed0_draw():
         assert r["dt"] == 0.2 * min(h * h * r["Re"] / 4.0, h)
         assert r["beta"] == cnv.reference_beta(r["n"])
     assert rows == cnv.batch_table()
+
+
+def test_params_layout_matches_library():
+    import ctypes
+    import cnv
+    L = cnv.lib()
+    assert L.cnv_params_size() == ctypes.sizeof(cnv.Params)

@@ -75,7 +75,7 @@ def test_batch_autotune_keeps_results():
     with cnv.Batch(ps) as b:
         shape = b.autotune(launches=6)
         print("autotuned", shape, flush=True)
-        assert shape["sweeps_per_launch"] in (1, 2, 4, 6, 8) and shape["ms_per_sweep"] > 0
+        assert shape["sweeps_per_launch"] in (1, 2, 4, 5, 6, 8) and shape["ms_per_sweep"] > 0
         bst = b.step(2)
         got = [b.get_state(i) for i in range(len(ps))]
     for i, p in enumerate(ps):

@@ -95,6 +95,8 @@ CASES = [  # n, order, poisson_type, beta, spl, Re, dt, tol, steps, bc
     (64, 2, 2, None, 8, 1000.0, 0.002, 1e-10, 3, 1),
     (97, 6, 2, 1.9, 4, 3000.0, 0.001, 1e-10, 2, 1),
     (130, 6, 2, None, 8, 1000.0, 0.0005, 1e-9, 2, 0),
+    (150, 4, 2, 1.8, 5, 500.0, 0.001, 1e-10, 2, 1),   # 16 waves x 4 rows, 5 sweeps per launch
+    (71, 6, 1, None, 5, 200.0, 0.002, 1e-10, 2, 0),
 ]
 
 
