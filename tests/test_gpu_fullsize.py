@@ -1,0 +1,69 @@
+"""GPU tests at the workload's full size and beyond, where the oracle cannot run whole steps:
+size-independent properties of a 1024^2 step (the metric's cavity), and bit-exact Poisson
+iterates on 4096^2-class grids (thousands of workgroups, several per CU, both access paths)."""
+import numpy as np
+import pytest
+
+import cnv
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+N, RE, DT, OMEGA, TOL = 1024, 1000.0, 50 * 2.0 ** -20, 1.95, 1e-10
+
+
+def rb_sweep_update(psi, f, dx, dy, beta):
+    """One red-black SOR sweep (src/poisson.c:268, red points first); returns the L1 update."""
+    p = psi.copy()
+    c1, c2, c3 = dy * dy, dx * dx, dx * dx * dy * dy
+    B = 2 * (dx * dx + dy * dy)
+    i, j = np.meshgrid(np.arange(1, p.shape[0] - 1), np.arange(1, p.shape[1] - 1), indexing="ij")
+    total = 0.0
+    for colour in (0, 1):
+        m = ((i + j) & 1) == colour
+        ii, jj = i[m], j[m]
+        old = p[ii, jj].copy()
+        new = beta * (c1 * (p[ii + 1, jj] + p[ii - 1, jj]) + c2 * (p[ii, jj + 1] + p[ii, jj - 1])
+                      - c3 * f[ii, jj]) / B + (1 - beta) * old
+        p[ii, jj] = new
+        total += float(np.sum(np.abs(new - old)))
+    return total
+
+
+def test_fullsize_step_properties():
+    kw = dict(Re=RE, dt=DT, order=6, beta=OMEGA, tol=TOL, itmax=50_000_000)
+    print("1024^2 FP64 step", flush=True)
+    with cnv.Cavity(cnv.default_params(N, **kw)) as cav:
+        st = cav.step(1)[0]
+        s64 = cav.get_state()
+    # the returned iterate is the first with a sweep update below tol: one more sweep moves it
+    # by a little less again (the asymptotic SOR contraction), never by more than tol
+    nxt = rb_sweep_update(s64["psi"], -s64["w"], 1.0 / N, 1.0 / N, OMEGA)
+    print("sweeps", st.sweeps, "last update", st.e, "next update", nxt, flush=True)
+    assert st.status == cnv.OK and st.e < TOL
+    assert 0.5 * st.e < nxt < TOL
+    # walls of psi stay exactly 0; continuity stays at round-off
+    assert not np.any(s64["psi"][[0, -1], :]) and not np.any(s64["psi"][:, [0, -1]])
+    assert max(abs(st.cont_max), abs(st.cont_min)) < 1e-8
+    print("1024^2 FP32-refined step", flush=True)
+    with cnv.Cavity(cnv.default_params(N, precision=cnv.PREC_FP32_DC, **kw)) as cav:
+        sd = cav.step(1)[0]
+        s32 = cav.get_state()
+    assert sd.status == cnv.OK and sd.outer >= 2
+    for f in ("psi", "w", "u", "v"):
+        err = np.max(np.abs(s32[f] - s64[f])) / np.max(np.abs(s64[f]))
+        assert err <= 1e-6, (f, err)
+
+
+@pytest.mark.parametrize("n", [4096, 4099])
+def test_large_grid_poisson_iterates_bit_exact(n):
+    # 7 sweeps on a random source: 4096 takes the paired-access, exact-reciprocal path, 4099 the
+    # per-column, corrected-division path; both need thousands of workgroups
+    f = np.random.default_rng(n).standard_normal((n, n))
+    print("poisson n=%d" % n, flush=True)
+    with cnv.Cavity(cnv.default_params(n, tol=1e-300, itmax=7, diagnostics=0)) as cav:
+        rc, psi, k, e = cav.poisson(f)
+    assert rc == cnv.NOT_CONVERGED
+    psi_or, st_or = orc.poisson(orc.make_params(n, tol=1e-300, itmax=7, ordering=orc.REDBLACK, threads=8), f)
+    assert st_or.sweeps == 7
+    assert np.array_equal(psi, psi_or)
