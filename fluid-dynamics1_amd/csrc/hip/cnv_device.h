@@ -135,8 +135,5 @@ int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCt
                       int ld, hipStream_t st);
 int dc_inner_init_launch(float* e0, SorCtl* ctl, int n, int ld, double rtol, double tol,
                          int itmax, hipStream_t st);
-int zero_f32_launch(float* p, long long count, hipStream_t st);
-int velocity_fp32_launch(const double* psi, double* u, double* v, int n, int ld, const Stencils& s,
-                         hipStream_t st);
 
 }  // namespace cnv

@@ -13,8 +13,8 @@
 // solver's iteration and their update norm is its stopping quantity.  An inner solve ends early
 // ("fold") once its update norm fell below inner_rtol times the L1 norm of e, where FP32 stops
 // resolving the update; the correction is then folded into psi64 and the residual refreshed in
-// FP64, which restores full resolution.  Only the residual, the psi accumulation and the omega
-// update are FP64; the transport and velocity stencils run in FP32.
+// FP64, which restores full resolution.  The residual, the psi accumulation, the omega update and
+// the velocities (small differences of a large psi) are FP64; the transport stencils run in FP32.
 #include <float.h>
 
 #include "cnv_device.h"
@@ -56,11 +56,6 @@ __global__ void __launch_bounds__(kBlock) dc_inner_init_kernel(float* e0, SorCtl
     if (blockIdx.x == 0 && threadIdx.x < 2) ctl_reset(ctl + threadIdx.x, tol, rtol, itmax);
 }
 
-__global__ void __launch_bounds__(kBlock) zero_f32_kernel(float* p, long long count) {
-    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (q < count) p[q] = 0.0f;
-}
-
 __global__ void __launch_bounds__(kBlock) transport_fp32_kernel(const double* __restrict__ w,
                                                                double* __restrict__ wout,
                                                                const double* __restrict__ u,
@@ -70,18 +65,6 @@ __global__ void __launch_bounds__(kBlock) transport_fp32_kernel(const double* __
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i < n && j < n) transport_fp32_at(w, wout, u, v, n, ld, s, b, rRe, dt, i, j);
-}
-
-// u = DY psi, v = -(DX psi) with FP32 stencil arithmetic on the FP64 psi
-__global__ void __launch_bounds__(kBlock) velocity_fp32_kernel(const double* __restrict__ psi,
-                                                              double* __restrict__ u,
-                                                              double* __restrict__ v, int n, int ld,
-                                                              Stencils s) {
-    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= n || j >= n) return;
-    u[(size_t)i * ld + j] = (double)fsum_y(s.f1y, s.start, psi, i, j, ld);
-    v[(size_t)i * ld + j] = -(double)fsum_x(s.f1x, s.start, psi, i, j, ld);
 }
 
 static inline int launched() { return hipGetLastError() == hipSuccess ? 0 : 1; }
@@ -110,22 +93,10 @@ int dc_inner_init_launch(float* e0, SorCtl* ctl, int n, int ld, double rtol, dou
     return launched();
 }
 
-int zero_f32_launch(float* p, long long count, hipStream_t st) {
-    hipLaunchKernelGGL(zero_f32_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, count);
-    return launched();
-}
-
 int transport_fp32_launch(const double* w_in, double* w_out, const double* u, const double* v, int n, int ld,
                           const Stencils& s, const WallBC& bc, double rRe, double dt, hipStream_t st) {
     dim3 grid((n + 63) / 64, (n + 3) / 4);
     hipLaunchKernelGGL(transport_fp32_kernel, grid, dim3(kBlock), 0, st, w_in, w_out, u, v, n, ld, s, bc, rRe, dt);
-    return launched();
-}
-
-int velocity_fp32_launch(const double* psi, double* u, double* v, int n, int ld, const Stencils& s,
-                         hipStream_t st) {
-    dim3 grid((n + 63) / 64, (n + 3) / 4);
-    hipLaunchKernelGGL(velocity_fp32_kernel, grid, dim3(kBlock), 0, st, psi, u, v, n, ld, s);
     return launched();
 }
 

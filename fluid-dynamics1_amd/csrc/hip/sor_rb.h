@@ -4,10 +4,10 @@
 // One launch advances the whole grid by up to S full sweeps (red half-sweep, then black),
 // computing exactly what S successive global sweeps compute, point for point:
 //   * each workgroup owns a TI x TJ tile and loads it with a halo of H = 2S cells (one cell per
-//     half-sweep) into registers; half-sweep h updates the points of its colour inside the
-//     extended tile shrunk by h+1 on every side, whose four neighbours were all updated
-//     correctly by half-sweep h-1; after 2S half-sweeps the owned tile is exact and is written to
-//     the other buffer (ping-pong);
+//     half-sweep) into registers; after half-sweep h the points of the extended tile at distance
+//     > h from its edge hold exactly what h+1 global half-sweeps produce (their four neighbours
+//     were exact after half-sweep h-1); after 2S half-sweeps the owned tile is exact and is
+//     written to the other buffer (ping-pong);
 //   * the extended tile is 128 columns wide: lane m of every wave holds columns 2m and 2m+1,
 //     and wave w holds RB consecutive rows.  A point's N/S neighbours are in the same lane's
 //     registers (rows r+-1), one of its E/W neighbours is the lane's other column and the other
@@ -37,8 +37,9 @@
 //     discarded: this launch redoes s+1 sweeps from its (untouched) input buffer, so the final
 //     iterate is exactly the one of the first sweep with e < tol (src/poisson.c:273);
 //   * the sweep budget itmax is honoured exactly (src/poisson.c:234);
-//   * defect correction (FP32 planes): the solve also stops ("fold") once the update norm fell
-//     below rtol times the iterate's own L1 norm, the resolution limit of FP32.
+//   * relative rule: with rtol > 0 the solve also stops ("fold") once the update norm fell below
+//     rtol times the iterate's own L1 norm -- the resolution limit of an FP32 defect-correction
+//     inner solve, or the relative tolerance of the adjoint problem's FP64 solves (NORM).
 #pragma once
 
 #include <type_traits>
