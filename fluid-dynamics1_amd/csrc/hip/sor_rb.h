@@ -29,13 +29,15 @@
 // power of two (dx = dy = 2^-k), 1/B is exact and x*(1/B) is already RN(x/B): one multiply (XR).
 // The L1 norm of each sweep's update (src/poisson.c:34-60; sqrt(pow(d,2)) == |d|) is summed
 // over owned points per workgroup in a fixed order into partial[L&1][s][tile].  The next launch
-// decides on them in its prologue, overlapped with its own (speculative) tile load: every
-// workgroup reduces the partials in tile order, so all reach the same decision, and workgroup 0
-// (tile 0) records it in the state slot L&1 (SorCtl) that the host polls:
+// decides on them after its own (speculative) sweeps, so that neither the partials' latency nor
+// the reduction delays the sweeps: every workgroup reduces the partials in the same fixed order,
+// so all reach the same decision, and tile 0 records it in the state slot L&1 (SorCtl) that the
+// host polls:
 //   * a launch whose previous state is terminal copies it forward and returns;
+//   * if the previous launch ended the solve, this launch's speculative sweeps are discarded;
 //   * if sweep s < S-1 of the previous launch reached e < tol, that launch's output is
-//     discarded: this launch redoes s+1 sweeps from its (untouched) input buffer, so the final
-//     iterate is exactly the one of the first sweep with e < tol (src/poisson.c:273);
+//     discarded too: this launch redoes s+1 sweeps from its (untouched) input buffer, so the
+//     final iterate is exactly the one of the first sweep with e < tol (src/poisson.c:273);
 //   * the sweep budget itmax is honoured exactly (src/poisson.c:234);
 //   * relative rule: with rtol > 0 the solve also stops ("fold") once the update norm fell below
 //     rtol times the iterate's own L1 norm -- the resolution limit of an FP32 defect-correction
@@ -75,23 +77,24 @@ __device__ inline double sor_div(double x, double B, double rB) {
 __device__ inline float sor_div(float x, float, float rB) { return x * rB; }
 
 // The lane's column pair (gj, gj+1) of one row, gj even.  vec (wave-uniform, checked by the
-// host: nc and ld even, buffers 2*sizeof(T)-aligned): one 2-element access whenever the pair lies
-// inside the row (pair), with the columns outside in0 / in1 read as 0 or left unwritten.
-// Otherwise per column, dereferencing only columns inside in0 / in1.
+// host: nc and ld even, buffers 2*sizeof(T)-aligned): one 2-element access, the pair's when it
+// lies inside the row (pair), with the columns outside in0 / in1 read as 0 or left unwritten.
+// Otherwise per column.  Loads outside the data fetch element 0 instead and discard it.
 template <typename T>
 __device__ inline void load_pair(const T* __restrict__ p, long long off, bool pair, bool in0, bool in1,
                                  bool vec, T& x0, T& x1) {
     typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type V2;
+    // branch-free: lanes without data read element 0 (always valid) and discard it, so no
+    // divergent branch makes the compiler wait for each load before the next one is issued
     if (vec) {
-        V2 d;
-        d.x = T(0);
-        d.y = T(0);
-        if (pair) d = *reinterpret_cast<const V2*>(p + off);
+        const V2 d = *reinterpret_cast<const V2*>(p + (pair ? off : 0));
         x0 = in0 ? d.x : T(0);
         x1 = in1 ? d.y : T(0);
     } else {
-        x0 = in0 ? p[off] : T(0);
-        x1 = in1 ? p[off + 1] : T(0);
+        const T d0 = p[in0 ? off : 0];
+        const T d1 = p[in1 ? off + 1 : 0];
+        x0 = in0 ? d0 : T(0);
+        x1 = in1 ? d1 : T(0);
     }
 }
 template <typename T>
@@ -143,10 +146,10 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     typedef typename std::conditional<sizeof(T) == 4, float, double>::type Acc;
 
     __shared__ T xch[2][NW][2][64];  // [half-sweep parity][wave][first/last row][lane]
-    // one column per thread: the previous launch's partials (prologue), then this launch's
-    // per-lane L1 sums (sweep loop)
+    // one column per thread: this launch's per-lane L1 sums, one row per sweep; then the
+    // previous launch's partials for the decision
     __shared__ double lred[S][2][NW * 64];
-    __shared__ double etot[kMaxSweepsPerLaunch][2];
+    __shared__ double etot[S][2];       // the previous launch's partial totals
 
     const int L = a.launch;
     const SorCtl* prev = a.ctl + ((L + 1) & 1);
@@ -209,18 +212,9 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         }
     }
 
-    const int pstatus = prev->status;
-    if (pstatus != SOR_RUNNING) {  // the solve already ended: carry its state forward
-        if (tile == 0 && tid == 0) *mine = *prev;
-        return;
-    }
-    const int sd = prev->s_done;   // sweeps the previous launch performed (0: first launch)
-    const int k_prev = prev->k;
-    const int itmax = prev->itmax;
-    const double tol = prev->tol, rtol = prev->rtol;
-
     // the previous launch's per-tile partials: thread t sums tiles t, t + NT, ... in that order
-    // (all S slots are read; only the first sd are used)
+    // (all S slots are read, before the state slot is known, so that these loads are in flight
+    // together with the tile and the slot; only the first sd are used)
     const size_t pstride = (size_t)kMaxSweepsPerLaunch * ntiles * 2;
     double pv[S][2];
 #pragma unroll
@@ -235,9 +229,104 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
             }
         }
     }
-    // ---- decision on the previous launch: every workgroup reduces its per-tile partials in
-    //      the same fixed order and reaches the same decision (src/poisson.c:272-278) --------
+    const int pstatus = prev->status;
+    if (pstatus != SOR_RUNNING) {  // the solve already ended: carry its state forward
+        if (tile == 0 && tid == 0) *mine = *prev;
+        return;
+    }
+    const int sd = prev->s_done;   // sweeps the previous launch performed (0: first launch)
+    const int k_prev = prev->k;
+    const int itmax = prev->itmax;
+    const double tol = prev->tol, rtol = prev->rtol;
+    const int nq = kNorm ? 2 : 1;  // reduced quantities: L1 update (and the iterate's L1 norm)
+
+    // Sweep first, decide afterwards.  Unless the previous launch ended the solve, this launch
+    // performs s_spec = min(S, itmax - (k_prev + sd)) sweeps from the previous output, which the
+    // slot alone determines; the partials the decision needs have long arrived by the time the
+    // sweeps end.  If the decision then ends the solve, the sweeps are discarded; if the solve
+    // converged inside the previous launch (rollback), the tile is reloaded from that launch's
+    // untouched input buffer and swept again (second pass).
+    const int budget = itmax - (k_prev + sd);
+    const int s_spec = budget < S ? (budget > 0 ? budget : 0) : S;
+    const T c1 = a.c1, c2 = a.c2, B = a.B, rB = a.rB, omb = a.omb;
+    // S-sweep body (the speculative pass, and again for a rollback)
+    auto sweeps = [&](const int ns) {
+        // half-sweep 0 updates colour 0 and reads its neighbours' colour-1 boundary rows:
+        // colour 1 sits in column q = (r + 1) & 1 of row r
+        xch[1][wv][0][lane] = v[1][0];
+        xch[1][wv][1][lane] = v[(RB - 1 + 1) & 1][RB - 1];
+        __syncthreads();
+
+        for (int s = 0; s < ns; ++s) {
+            Acc acc[2] = {Acc(0), Acc(0)}, nrm[2] = {Acc(0), Acc(0)};
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                // boundary rows of the neighbouring waves (colour 1-c, written by half-sweep h-1)
+                const T sb = xch[c ^ 1][wv > 0 ? wv - 1 : 0][1][lane];
+                const T nb = xch[c ^ 1][wv < NW - 1 ? wv + 1 : NW - 1][0][lane];
+                // rows 1..RB-2 first: they do not wait for the LDS reads of sb / nb
+#pragma unroll
+                for (int rr = 0; rr < RB; ++rr) {
+                    const int r = rr < RB - 2 ? rr + 1 : (rr == RB - 2 ? 0 : RB - 1);
+                    const int q = (c + r) & 1;  // column of colour c in row r (compile-time)
+                    const T old = v[q][r];
+                    const T up = (r < RB - 1) ? v[q][r + 1] : nb;
+                    const T dn = (r > 0) ? v[q][r - 1] : sb;
+                    // E + W: the lane's other column plus the neighbouring lane's nearer column
+                    const T own = v[q ^ 1][r];
+                    const T far = q ? dpp_shift<kDppFromNext>(v[0][r]) : dpp_shift<kDppFromPrev>(v[1][r]);
+                    T cand;
+                    if constexpr (sizeof(T) == 8) {
+                        // the reference's association: ((c1*(N+S) + c2*(E+W)) - c3*f), times beta,
+                        // divided by B, plus (1-beta)*u0; + is commutative, so E/W order is free
+                        const T x = c1 * (up + dn) + c2 * (own + far) - fc[q][r];
+                        const T y = be[q][r] * x;
+                        const T qt = XR ? y * rB : sor_div(y, B, rB);
+                        cand = GS ? qt : qt + omb * old;
+                        v[q][r] = cand;
+                        acc[q] = __fma_rn(w[r], fabs(cand - old), acc[q]);
+                        if (kNorm) nrm[q] = __fma_rn(w[r], fabs(cand), nrm[q]);
+                    } else {
+                        const T x = __fmaf_rn(c1, up + dn, __fmaf_rn(c2, own + far, fc[q][r]));
+                        const T y = be[q][r] * x;
+                        cand = GS ? y : __fmaf_rn(omb, old, y);
+                        v[q][r] = cand;
+                        acc[q] = __fmaf_rn(w[r], fabsf(cand - old), acc[q]);
+                        nrm[q] = __fmaf_rn(w[r], fabsf(cand), nrm[q]);
+                    }
+                }
+                xch[c][wv][0][lane] = v[c & 1][0];
+                xch[c][wv][1][lane] = v[(c + RB - 1) & 1][RB - 1];
+                __syncthreads();
+            }
+            // per-lane sums of this sweep, owned columns only; reduced once after the last sweep
+            lred[s][0][tid] = (double)((ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0)));
+            if (kNorm)
+                lred[s][1][tid] = (double)((ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0)));
+        }
+    };
+    int mode = 0;          // 0: sweep on, 1: stop (terminal), 2: redo s_run sweeps (rollback)
+    int s_run = s_spec;
+    sweeps(s_spec);
+
+    // this launch's per-tile partials (the sweeps' per-lane sums in lred): wave w reduces
+    // (sweep, quantity) rows, in a fixed order.  Written whatever the decision below: a launch
+    // that ends the solve leaves partials nobody reads.
+    __syncthreads();
+    for (int row = wv; row < nq * s_spec; row += NW) {
+        const int s = row / nq, q = row - s * nq;
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < NW; ++j) acc += lred[s][q][lane + 64 * j];
+        acc = wave_sum(acc);
+        if (lane == 0) a.partial[(L & 1) * pstride + ((size_t)s * ntiles + tile) * 2 + q] = acc;
+    }
+
+    // ---- decision on the previous launch: every workgroup reduces the per-tile partials in
+    //      the same fixed order (lred reused, one column per thread) and reaches the same
+    //      decision (src/poisson.c:272-278) ---------------------------------------------------
     if (sd > 0) {
+        __syncthreads();  // every wave is done reading lred above
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             if (s < sd) {
@@ -246,9 +335,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
             }
         }
         __syncthreads();
-        // wave w reduces (sweep, quantity) rows; without kNorm there is no iterate norm
-        const int nq = kNorm ? 2 : 1;
-        if (nq == 1 && tid < kMaxSweepsPerLaunch) etot[tid][1] = 0.0;
+        if (nq == 1 && tid < S) etot[tid][1] = 0.0;
         for (int row = wv; row < nq * sd; row += NW) {
             const int s = row / nq, q = row - s * nq;
             double acc = 0.0;
@@ -259,8 +346,6 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         }
         __syncthreads();
     }
-    int mode = 0;          // 0: sweep on, 1: stop (terminal), 2: redo s_run sweeps (rollback)
-    int s_run = 0;
     SorCtl next;
     next.k = k_prev;
     next.parity = L & 1;   // where the previous launch left its output
@@ -284,8 +369,8 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
             }
         }
         if (s_conv >= 0 && s_conv < sd - 1) {
-            // converged inside the previous launch: redo s_conv+1 sweeps from its (untouched)
-            // input buffer; that iterate is the final one (src/poisson.c:273)
+            // converged inside the previous launch: redo s_conv+1 sweeps from its
+            // (untouched) input buffer; that iterate is the final one (src/poisson.c:273)
             mode = 2;
             s_run = s_conv + 1;
             next.status = SOR_CONVERGED;
@@ -311,16 +396,11 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         next.status = SOR_FAILED;
         mode = 1;
     }
-    if (mode == 0) s_run = (itmax - next.k) < S ? (itmax - next.k) : S;
-    next.s_done = (mode == 0) ? s_run : 0;
+    next.s_done = (mode == 0) ? s_run : 0;  // mode 0: s_run == s_spec, the sweeps just done
     if (mode != 1) next.launches += 1;
     if (tile == 0 && tid == 0) *mine = next;
-    if (mode == 1) return;
-    const int s_eff = s_run;
-    // normal: read the previous launch's output, write the other buffer; rollback: read the
-    // previous launch's input, write over its output
-    T* __restrict__ dst = ((L & 1) ^ (mode == 2 ? 0 : 1)) ? a.psi1 : a.psi0;
     if (mode == 2) {
+        // rollback: the previous launch's input, swept s_run times, replaces its output
         const T* __restrict__ srcR = (L & 1) ? a.psi0 : a.psi1;
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
@@ -329,62 +409,13 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
             load_pair(srcR, (long long)(rin ? gi : 0) * ld + gjq[0], rin && colv[0], rin && colv[0],
                       rin && colv[1], vec, v[0][r], v[1][r]);
         }
+        __syncthreads();  // every wave is done reading xch of the first pass
+        sweeps(s_run);
     }
-
-    // half-sweep 0 updates colour 0 and reads its neighbours' colour-1 boundary rows:
-    // colour 1 sits in column q = (r + 1) & 1 of row r
-    xch[1][wv][0][lane] = v[1][0];
-    xch[1][wv][1][lane] = v[(RB - 1 + 1) & 1][RB - 1];
-    __syncthreads();  // also: every wave is done reading lred in the prologue
-
-    const T c1 = a.c1, c2 = a.c2, B = a.B, rB = a.rB, omb = a.omb;
-    for (int s = 0; s < s_eff; ++s) {
-        Acc acc[2] = {Acc(0), Acc(0)}, nrm[2] = {Acc(0), Acc(0)};
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            // boundary rows of the neighbouring waves (colour 1-c, written by half-sweep h-1)
-            const T sb = xch[c ^ 1][wv > 0 ? wv - 1 : 0][1][lane];
-            const T nb = xch[c ^ 1][wv < NW - 1 ? wv + 1 : NW - 1][0][lane];
-            // rows 1..RB-2 first: they do not wait for the LDS reads of sb / nb
-#pragma unroll
-            for (int rr = 0; rr < RB; ++rr) {
-                const int r = rr < RB - 2 ? rr + 1 : (rr == RB - 2 ? 0 : RB - 1);
-                const int q = (c + r) & 1;  // column of colour c in row r (compile-time)
-                const T old = v[q][r];
-                const T up = (r < RB - 1) ? v[q][r + 1] : nb;
-                const T dn = (r > 0) ? v[q][r - 1] : sb;
-                // E + W: the lane's other column plus the neighbouring lane's nearer column
-                const T own = v[q ^ 1][r];
-                const T far = q ? dpp_shift<kDppFromNext>(v[0][r]) : dpp_shift<kDppFromPrev>(v[1][r]);
-                T cand;
-                if constexpr (sizeof(T) == 8) {
-                    // the reference's association: ((c1*(N+S) + c2*(E+W)) - c3*f), times beta,
-                    // divided by B, plus (1-beta)*u0; + is commutative, so E/W order is free
-                    const T x = c1 * (up + dn) + c2 * (own + far) - fc[q][r];
-                    const T y = be[q][r] * x;
-                    const T qt = XR ? y * rB : sor_div(y, B, rB);
-                    cand = GS ? qt : qt + omb * old;
-                    v[q][r] = cand;
-                    acc[q] = __fma_rn(w[r], fabs(cand - old), acc[q]);
-                    if (kNorm) nrm[q] = __fma_rn(w[r], fabs(cand), nrm[q]);
-                } else {
-                    const T x = __fmaf_rn(c1, up + dn, __fmaf_rn(c2, own + far, fc[q][r]));
-                    const T y = be[q][r] * x;
-                    cand = GS ? y : __fmaf_rn(omb, old, y);
-                    v[q][r] = cand;
-                    acc[q] = __fmaf_rn(w[r], fabsf(cand - old), acc[q]);
-                    nrm[q] = __fmaf_rn(w[r], fabsf(cand), nrm[q]);
-                }
-            }
-            xch[c][wv][0][lane] = v[c & 1][0];
-            xch[c][wv][1][lane] = v[(c + RB - 1) & 1][RB - 1];
-            __syncthreads();
-        }
-        // per-lane sums of this sweep, owned columns only; reduced once after the last sweep
-        lred[s][0][tid] = (double)((ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0)));
-        if (kNorm)
-            lred[s][1][tid] = (double)((ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0)));
-    }
+    if (mode == 1) return;
+    // normal: the previous launch's output was read, the other buffer is written; rollback: the
+    // previous launch's input was read, its output is overwritten
+    T* __restrict__ dst = ((L & 1) ^ (mode == 2 ? 0 : 1)) ? a.psi1 : a.psi0;
 
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
@@ -396,18 +427,6 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
             store_pair(dst, (long long)gi * ld + gjq[0], ownc[0], ownc[1], vec,
                        (rint && colin[0]) ? v[0][r] : T(0), (rint && colin[1]) ? v[1][r] : T(0));
         }
-    }
-    if (mode != 0) return;  // a rollback launch ends the solve: no partials needed
-    __syncthreads();
-    // this launch's per-tile partials: wave w reduces (sweep, quantity) rows, in a fixed order
-    const int nq = kNorm ? 2 : 1;
-    for (int row = wv; row < nq * s_eff; row += NW) {
-        const int s = row / nq, q = row - s * nq;
-        double acc = 0.0;
-#pragma unroll
-        for (int j = 0; j < NW; ++j) acc += lred[s][q][lane + 64 * j];
-        acc = wave_sum(acc);
-        if (lane == 0) a.partial[(L & 1) * pstride + ((size_t)s * ntiles + tile) * 2 + q] = acc;
     }
 }
 
