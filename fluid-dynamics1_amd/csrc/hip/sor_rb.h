@@ -50,13 +50,6 @@
 
 namespace cnv {
 
-template <typename T>
-__device__ inline T wave_sum(T x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
-}
-
 __device__ inline int sor_tile_of_block(int b, int ntiles) {
     // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X.  Give each XCD a
     // contiguous band of tiles so that the halo rows two neighbouring tiles share are fetched
@@ -114,8 +107,9 @@ __device__ inline void store_pair(T* __restrict__ p, long long off, bool in0, bo
     }
 }
 
-// Lane i receives lane i+1's value (wave_shl:1) / lane i-1's value (wave_shr:1); the lane
-// without a source (63 / 0) reads 0 (bound_ctrl), so no old value has to be materialised.
+// Lane i receives the value of the lane the DPP control CTRL names, e.g. lane i+1's
+// (wave_shl:1) or lane i-1's (wave_shr:1); a lane without a source (63 / 0 there) reads 0
+// (bound_ctrl), so no old value has to be materialised.
 template <int CTRL>
 __device__ inline double dpp_shift(double x) {
     const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, true);
@@ -128,6 +122,23 @@ __device__ inline float dpp_shift(float x) {
 }
 constexpr int kDppFromNext = 0x130;  // wave_shl:1
 constexpr int kDppFromPrev = 0x138;  // wave_shr:1
+
+// Sum over the 64 lanes of a wave, returned to every lane, in a fixed association (the same in
+// every workgroup): pairs and quads (quad_perm [1,0,3,2] and [2,3,0,1]), the four quads of each
+// 16-lane row (row_shr 4 and 8: lane 15 of a row holds the row's sum), then the rows
+// (row_bcast 15: lane 63 gets rows 3 + 2, lane 31 rows 1 + 0; row_bcast 31: lane 63 gets all
+// four); lane 63's value is broadcast.  Cross-lane moves only, no LDS round trips.
+__device__ inline double wave_sum(double x) {
+    x += dpp_shift<0xb1>(x);
+    x += dpp_shift<0x4e>(x);
+    x += dpp_shift<0x114>(x);
+    x += dpp_shift<0x118>(x);
+    x += dpp_shift<0x142>(x);
+    x += dpp_shift<0x143>(x);
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
+    return __hiloint2double(hi, lo);
+}
 
 // One workgroup's launch on tile `tile` of the solve described by `a` (NW*64 threads).
 template <typename T, int S, int NW, int RB, bool GS, bool XR, bool NORM>
