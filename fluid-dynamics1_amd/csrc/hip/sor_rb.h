@@ -128,16 +128,35 @@ constexpr int kDppFromPrev = 0x138;  // wave_shr:1
 // 16-lane row (row_shr 4 and 8: lane 15 of a row holds the row's sum), then the rows
 // (row_bcast 15: lane 63 gets rows 3 + 2, lane 31 rows 1 + 0; row_bcast 31: lane 63 gets all
 // four); lane 63's value is broadcast.  Cross-lane moves only, no LDS round trips.
+template <int CTRL, int N>
+__device__ inline void dpp_accumulate(double (&x)[N]) {
+    double y[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) y[i] = dpp_shift<CTRL>(x[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] += y[i];
+}
+// N independent sums, each step applied to all of them before the next, so that their chains
+// interleave (no idle cycles between a value's update and its next cross-lane read)
+template <int N>
+__device__ inline void wave_sum_n(double (&x)[N]) {
+    dpp_accumulate<0xb1>(x);
+    dpp_accumulate<0x4e>(x);
+    dpp_accumulate<0x114>(x);
+    dpp_accumulate<0x118>(x);
+    dpp_accumulate<0x142>(x);
+    dpp_accumulate<0x143>(x);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const int lo = __builtin_amdgcn_readlane(__double2loint(x[i]), 63);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(x[i]), 63);
+        x[i] = __hiloint2double(hi, lo);
+    }
+}
 __device__ inline double wave_sum(double x) {
-    x += dpp_shift<0xb1>(x);
-    x += dpp_shift<0x4e>(x);
-    x += dpp_shift<0x114>(x);
-    x += dpp_shift<0x118>(x);
-    x += dpp_shift<0x142>(x);
-    x += dpp_shift<0x143>(x);
-    const int lo = __builtin_amdgcn_readlane(__double2loint(x), 63);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 63);
-    return __hiloint2double(hi, lo);
+    double v[1] = {x};
+    wave_sum_n(v);
+    return v[0];
 }
 
 // One workgroup's launch on tile `tile` of the solve described by `a` (NW*64 threads).
@@ -157,10 +176,9 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     typedef typename std::conditional<sizeof(T) == 4, float, double>::type Acc;
 
     __shared__ T xch[2][NW][2][64];  // [half-sweep parity][wave][first/last row][lane]
-    // one column per thread: this launch's per-lane L1 sums, one row per sweep; then the
-    // previous launch's partials for the decision
+    // one column per thread: this launch's per-lane L1 sums, one row per sweep
     __shared__ double lred[S][2][NW * 64];
-    __shared__ double etot[S][2];       // the previous launch's partial totals
+    __shared__ double lw[NW][S][2];     // per-wave totals of the previous launch's partials
 
     const int L = a.launch;
     const SorCtl* prev = a.ctl + ((L + 1) & 1);
@@ -249,7 +267,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     const int k_prev = prev->k;
     const int itmax = prev->itmax;
     const double tol = prev->tol, rtol = prev->rtol;
-    const int nq = kNorm ? 2 : 1;  // reduced quantities: L1 update (and the iterate's L1 norm)
+    constexpr int nq = kNorm ? 2 : 1;  // reduced quantities: L1 update (and the iterate's L1 norm)
 
     // Sweep first, decide afterwards.  Unless the previous launch ended the solve, this launch
     // performs s_spec = min(S, itmax - (k_prev + sd)) sweeps from the previous output, which the
@@ -260,70 +278,105 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     const int budget = itmax - (k_prev + sd);
     const int s_spec = budget < S ? (budget > 0 ? budget : 0) : S;
     const T c1 = a.c1, c2 = a.c2, B = a.B, rB = a.rB, omb = a.omb;
-    // S-sweep body (the speculative pass, and again for a rollback)
-    auto sweeps = [&](const int ns) {
-        // half-sweep 0 updates colour 0 and reads its neighbours' colour-1 boundary rows:
-        // colour 1 sits in column q = (r + 1) & 1 of row r
+    // half-sweep 0 updates colour 0 and reads its neighbours' colour-1 boundary rows: colour 1
+    // sits in column q = (r + 1) & 1 of row r
+    auto sweeps_begin = [&]() {
         xch[1][wv][0][lane] = v[1][0];
         xch[1][wv][1][lane] = v[(RB - 1 + 1) & 1][RB - 1];
         __syncthreads();
-
-        for (int s = 0; s < ns; ++s) {
-            Acc acc[2] = {Acc(0), Acc(0)}, nrm[2] = {Acc(0), Acc(0)};
+    };
+    // sweep s (red half-sweep, then black) of a pass
+    auto one_sweep = [&](const int s) {
+        Acc acc[2] = {Acc(0), Acc(0)}, nrm[2] = {Acc(0), Acc(0)};
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                // boundary rows of the neighbouring waves (colour 1-c, written by half-sweep h-1)
-                const T sb = xch[c ^ 1][wv > 0 ? wv - 1 : 0][1][lane];
-                const T nb = xch[c ^ 1][wv < NW - 1 ? wv + 1 : NW - 1][0][lane];
-                // rows 1..RB-2 first: they do not wait for the LDS reads of sb / nb
+        for (int c = 0; c < 2; ++c) {
+            // boundary rows of the neighbouring waves (colour 1-c, written by half-sweep h-1)
+            const T sb = xch[c ^ 1][wv > 0 ? wv - 1 : 0][1][lane];
+            const T nb = xch[c ^ 1][wv < NW - 1 ? wv + 1 : NW - 1][0][lane];
+            // rows 1..RB-2 first: they do not wait for the LDS reads of sb / nb
 #pragma unroll
-                for (int rr = 0; rr < RB; ++rr) {
-                    const int r = rr < RB - 2 ? rr + 1 : (rr == RB - 2 ? 0 : RB - 1);
-                    const int q = (c + r) & 1;  // column of colour c in row r (compile-time)
-                    const T old = v[q][r];
-                    const T up = (r < RB - 1) ? v[q][r + 1] : nb;
-                    const T dn = (r > 0) ? v[q][r - 1] : sb;
-                    // E + W: the lane's other column plus the neighbouring lane's nearer column
-                    const T own = v[q ^ 1][r];
-                    const T far = q ? dpp_shift<kDppFromNext>(v[0][r]) : dpp_shift<kDppFromPrev>(v[1][r]);
-                    T cand;
-                    if constexpr (sizeof(T) == 8) {
-                        // the reference's association: ((c1*(N+S) + c2*(E+W)) - c3*f), times beta,
-                        // divided by B, plus (1-beta)*u0; + is commutative, so E/W order is free
-                        const T x = c1 * (up + dn) + c2 * (own + far) - fc[q][r];
-                        const T y = be[q][r] * x;
-                        const T qt = XR ? y * rB : sor_div(y, B, rB);
-                        cand = GS ? qt : qt + omb * old;
-                        v[q][r] = cand;
-                        acc[q] = __fma_rn(w[r], fabs(cand - old), acc[q]);
-                        if (kNorm) nrm[q] = __fma_rn(w[r], fabs(cand), nrm[q]);
-                    } else {
-                        const T x = __fmaf_rn(c1, up + dn, __fmaf_rn(c2, own + far, fc[q][r]));
-                        const T y = be[q][r] * x;
-                        cand = GS ? y : __fmaf_rn(omb, old, y);
-                        v[q][r] = cand;
-                        acc[q] = __fmaf_rn(w[r], fabsf(cand - old), acc[q]);
-                        nrm[q] = __fmaf_rn(w[r], fabsf(cand), nrm[q]);
-                    }
+            for (int rr = 0; rr < RB; ++rr) {
+                const int r = rr < RB - 2 ? rr + 1 : (rr == RB - 2 ? 0 : RB - 1);
+                const int q = (c + r) & 1;  // column of colour c in row r (compile-time)
+                const T old = v[q][r];
+                const T up = (r < RB - 1) ? v[q][r + 1] : nb;
+                const T dn = (r > 0) ? v[q][r - 1] : sb;
+                // E + W: the lane's other column plus the neighbouring lane's nearer column
+                const T own = v[q ^ 1][r];
+                const T far = q ? dpp_shift<kDppFromNext>(v[0][r]) : dpp_shift<kDppFromPrev>(v[1][r]);
+                T cand;
+                if constexpr (sizeof(T) == 8) {
+                    // the reference's association: ((c1*(N+S) + c2*(E+W)) - c3*f), times beta,
+                    // divided by B, plus (1-beta)*u0; + is commutative, so E/W order is free
+                    const T x = c1 * (up + dn) + c2 * (own + far) - fc[q][r];
+                    const T y = be[q][r] * x;
+                    const T qt = XR ? y * rB : sor_div(y, B, rB);
+                    cand = GS ? qt : qt + omb * old;
+                    v[q][r] = cand;
+                    acc[q] = __fma_rn(w[r], fabs(cand - old), acc[q]);
+                    if (kNorm) nrm[q] = __fma_rn(w[r], fabs(cand), nrm[q]);
+                } else {
+                    const T x = __fmaf_rn(c1, up + dn, __fmaf_rn(c2, own + far, fc[q][r]));
+                    const T y = be[q][r] * x;
+                    cand = GS ? y : __fmaf_rn(omb, old, y);
+                    v[q][r] = cand;
+                    acc[q] = __fmaf_rn(w[r], fabsf(cand - old), acc[q]);
+                    nrm[q] = __fmaf_rn(w[r], fabsf(cand), nrm[q]);
                 }
-                xch[c][wv][0][lane] = v[c & 1][0];
-                xch[c][wv][1][lane] = v[(c + RB - 1) & 1][RB - 1];
-                __syncthreads();
             }
-            // per-lane sums of this sweep, owned columns only; reduced once after the last sweep
-            lred[s][0][tid] = (double)((ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0)));
-            if (kNorm)
-                lred[s][1][tid] = (double)((ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0)));
+            xch[c][wv][0][lane] = v[c & 1][0];
+            xch[c][wv][1][lane] = v[(c + RB - 1) & 1][RB - 1];
+            __syncthreads();
         }
+        // per-lane sums of this sweep, owned columns only; reduced once after the last sweep
+        lred[s][0][tid] = (double)((ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0)));
+        if (kNorm)
+            lred[s][1][tid] = (double)((ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0)));
+    };
+    // a pass of ns sweeps; the rollback pass (rare) and a budget-limited last launch take the loop
+    auto sweeps = [&](const int ns) {
+        sweeps_begin();
+        for (int s = 0; s < ns; ++s) one_sweep(s);
     };
     int mode = 0;          // 0: sweep on, 1: stop (terminal), 2: redo s_run sweeps (rollback)
     int s_run = s_spec;
-    sweeps(s_spec);
+    // the common full pass, unrolled: the values stay in their registers from sweep to sweep (a
+    // rolled loop makes the compiler rotate them through copies at every iteration); not for the
+    // FP64 shapes whose unrolled pass no longer fits the register budget of their wave count
+    constexpr bool kUnroll = sizeof(T) == 4 || (S * RB <= 40 && NW != 12 && !(kNorm && S * RB > 20));
+    if (kUnroll && s_spec == S) {
+        sweeps_begin();
+#pragma unroll
+        for (int s = 0; s < S; ++s) one_sweep(s);
+    } else {
+        sweeps(s_spec);
+    }
+
+    // ---- reductions: one barrier for both --------------------------------------------------
+    // The previous launch's partials: the waves whose threads hold some (thread t summed tiles
+    // t, t + NT, ...) reduce them per (sweep, quantity) across their lanes; the per-wave totals
+    // go to LDS.  A barrier waits for outstanding global stores too, so none is issued before it.
+    // All S slots are reduced (only the first sd are read): no branch separates the chains.
+    const int nwp = min(NW, (ntiles + 63) >> 6);  // waves holding partials
+    if (sd > 0 && wv < nwp) {
+        double t[S * nq];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int q = 0; q < nq; ++q) t[s * nq + q] = pv[s][q];
+        wave_sum_n(t);
+        if (lane == 0) {
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int q = 0; q < nq; ++q) lw[wv][s][q] = t[s * nq + q];
+        }
+    }
+    __syncthreads();  // this launch's lane sums (lred) and the per-wave totals (lw) are complete
 
     // this launch's per-tile partials (the sweeps' per-lane sums in lred): wave w reduces
     // (sweep, quantity) rows, in a fixed order.  Written whatever the decision below: a launch
     // that ends the solve leaves partials nobody reads.
-    __syncthreads();
     for (int row = wv; row < nq * s_spec; row += NW) {
         const int s = row / nq, q = row - s * nq;
         double acc = 0.0;
@@ -333,29 +386,26 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         if (lane == 0) a.partial[(L & 1) * pstride + ((size_t)s * ntiles + tile) * 2 + q] = acc;
     }
 
-    // ---- decision on the previous launch: every workgroup reduces the per-tile partials in
-    //      the same fixed order (lred reused, one column per thread) and reaches the same
-    //      decision (src/poisson.c:272-278) ---------------------------------------------------
-    if (sd > 0) {
-        __syncthreads();  // every wave is done reading lred above
+    // ---- decision on the previous launch (src/poisson.c:272-278): lane s*nq+q of every wave sums
+    //      the per-wave totals of (s, q) in wave order, the same in every workgroup, so all reach
+    //      the same decision; the totals are then broadcast to the whole wave -------------------
+    double etot[S][2];
+    {
+        double e = 0.0;
+        if (sd > 0 && lane < nq * sd) {
+            const int s = lane / nq, q = lane - s * nq;
+            for (int j = 0; j < nwp; ++j) e += lw[j][s][q];
+        }
+        const int lo = __double2loint(e), hi = __double2hiint(e);
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if (s < sd) {
-                lred[s][0][tid] = pv[s][0];
-                lred[s][1][tid] = pv[s][1];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                etot[s][q] = q < nq ? __hiloint2double(__builtin_amdgcn_readlane(hi, s * nq + q),
+                                                       __builtin_amdgcn_readlane(lo, s * nq + q))
+                                    : 0.0;
             }
         }
-        __syncthreads();
-        if (nq == 1 && tid < S) etot[tid][1] = 0.0;
-        for (int row = wv; row < nq * sd; row += NW) {
-            const int s = row / nq, q = row - s * nq;
-            double acc = 0.0;
-#pragma unroll
-            for (int j = 0; j < NW; ++j) acc += lred[s][q][lane + 64 * j];
-            acc = wave_sum(acc);
-            if (lane == 0) etot[s][q] = acc;
-        }
-        __syncthreads();
     }
     SorCtl next;
     next.k = k_prev;
@@ -368,15 +418,23 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     next.e_last = prev->e_last;
     next.n_last = prev->n_last;
     if (sd > 0) {
+        // the first sweep s < sd with e < tol (converged) or, failing that, below rtol times the
+        // iterate's norm (fold); static indices keep etot in registers
         int s_conv = -1, s_fold = -1;
-        for (int s = 0; s < sd; ++s) {
-            if (etot[s][0] < tol) {
-                s_conv = s;
-                break;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (s < sd && s_conv < 0 && s_fold < 0) {
+                if (etot[s][0] < tol) s_conv = s;
+                else if (rtol > 0.0 && etot[s][0] < rtol * etot[s][1]) s_fold = s;
             }
-            if (rtol > 0.0 && etot[s][0] < rtol * etot[s][1]) {
-                s_fold = s;
-                break;
+        }
+        const int s_rep = (s_conv >= 0 && s_conv < sd - 1) ? s_conv : sd - 1;  // reported sweep
+        double e_rep = 0.0, n_rep = 0.0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (s == s_rep) {
+                e_rep = etot[s][0];
+                n_rep = etot[s][1];
             }
         }
         if (s_conv >= 0 && s_conv < sd - 1) {
@@ -386,12 +444,12 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
             s_run = s_conv + 1;
             next.status = SOR_CONVERGED;
             next.k = k_prev + s_run;
-            next.e_last = etot[s_conv][0];
-            next.n_last = etot[s_conv][1];
+            next.e_last = e_rep;
+            next.n_last = n_rep;
         } else {
             next.k = k_prev + sd;
-            next.e_last = etot[sd - 1][0];
-            next.n_last = etot[sd - 1][1];
+            next.e_last = e_rep;
+            next.n_last = n_rep;
             if (s_conv >= 0) {
                 next.status = SOR_CONVERGED;
                 mode = 1;
