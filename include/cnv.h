@@ -124,7 +124,9 @@ int cnv_sub_host(int m, int n, const double *a, const double *b, double *out);
 /* Measurement hooks used by bench.py.
  * cnv_time_sor: replays `launches` SOR kernel launches (sweeps_per_launch sweeps each) on the
  * current f with convergence disabled and returns the mean launch duration in ms, measured with
- * HIP events on the stream the kernel runs on.  Restores nothing: call on a scratch context. */
+ * HIP events on the stream the kernel runs on, after 1000 untimed launches (the steady state of
+ * a solve); the timed launches are captured in a graph and replayed, as a solve's chunks are.
+ * Restores nothing: call on a scratch context. */
 int cnv_time_sor(cnv_ctx *ctx, int launches, double *mean_ms);
 int cnv_sweeps_per_launch(const cnv_ctx *ctx);
 /* Counter calibration: copies nbytes between two scratch device buffers with a plain copy kernel
