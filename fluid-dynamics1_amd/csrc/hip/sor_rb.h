@@ -39,9 +39,10 @@
 //     discarded too: this launch redoes s+1 sweeps from its (untouched) input buffer, so the
 //     final iterate is exactly the one of the first sweep with e < tol (src/poisson.c:273);
 //   * the sweep budget itmax is honoured exactly (src/poisson.c:234);
-//   * relative rule: with rtol > 0 the solve also stops ("fold") once the update norm fell below
-//     rtol times the iterate's own L1 norm -- the resolution limit of an FP32 defect-correction
-//     inner solve, or the relative tolerance of the adjoint problem's FP64 solves (NORM).
+//   * relative rule: with rtol > 0 the solve also stops ("fold") once the update norm of a sweep
+//     fell below rtol times the iterate's own L1 norm after that launch's last sweep (measured
+//     once per launch) -- the resolution limit of an FP32 defect-correction inner solve, or the
+//     relative tolerance of the adjoint problem's FP64 solves (NORM).
 #pragma once
 
 #include <type_traits>
@@ -175,7 +176,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
 #pragma clang fp contract(off)
     constexpr int H = 2 * S;
     // the iterate's L1 norm is needed by the relative stopping rule (rtol): always on FP32
-    // planes, on request (NORM) for FP64
+    // planes, on request (NORM) for FP64.  It is measured once per launch, after its last sweep.
     constexpr bool kNorm = sizeof(T) == 4 || NORM;
     constexpr int EJ = 128;          // extended tile columns; lane m holds columns 2m, 2m+1
     constexpr int TJ = EJ - 2 * H;
@@ -186,9 +187,12 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     typedef typename std::conditional<sizeof(T) == 4, float, double>::type Acc;
 
     __shared__ T xch[2][NW][2][64];  // [half-sweep parity][wave][first/last row][lane]
-    // one column per thread: this launch's per-lane L1 sums, one row per sweep
-    __shared__ double lred[S][2][NW * 64];
-    __shared__ double lw[NW][S][2];     // per-wave totals of the previous launch's partials
+    // one column per thread: this launch's per-lane L1 update sums, one row per sweep, and the
+    // per-lane iterate norm after the last sweep
+    __shared__ double lred[S][NW * 64];
+    __shared__ double lnrm[kNorm ? NW * 64 : 1];
+    constexpr int nq = S + (kNorm ? 1 : 0);  // reduced quantities: S updates (and the norm)
+    __shared__ double lw[NW][nq];       // per-wave totals of the previous launch's partials
 
     const int L = a.launch;
     const SorCtl* prev = a.ctl + ((L + 1) & 1);
@@ -253,19 +257,18 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
 
     // the previous launch's per-tile partials: thread t sums tiles t, t + NT, ... in that order
     // (all S slots are read, before the state slot is known, so that these loads are in flight
-    // together with the tile and the slot; only the first sd are used)
+    // together with the tile and the slot; only the first sd are used).  Slot layout per tile:
+    // [s][tile][0] the L1 update of sweep s, [0][tile][1] the iterate's norm after the last sweep.
     const size_t pstride = (size_t)kMaxSweepsPerLaunch * ntiles * 2;
-    double pv[S][2];
+    double pv[nq];  // pv[s]: updates; pv[S]: norm
 #pragma unroll
-    for (int s = 0; s < S; ++s) pv[s][0] = pv[s][1] = 0.0;
+    for (int i = 0; i < nq; ++i) pv[i] = 0.0;
     {
         const double* pp = a.partial + ((L + 1) & 1) * pstride;
         for (int t = tid; t < ntiles; t += NW * 64) {
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                pv[s][0] += pp[((size_t)s * ntiles + t) * 2];
-                if (kNorm) pv[s][1] += pp[((size_t)s * ntiles + t) * 2 + 1];
-            }
+            for (int s = 0; s < S; ++s) pv[s] += pp[((size_t)s * ntiles + t) * 2];
+            if (kNorm) pv[nq - 1] += pp[(size_t)t * 2 + 1];
         }
     }
     const int pstatus = prev->status;
@@ -277,7 +280,6 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     const int k_prev = prev->k;
     const int itmax = prev->itmax;
     const double tol = prev->tol, rtol = prev->rtol;
-    constexpr int nq = kNorm ? 2 : 1;  // reduced quantities: L1 update (and the iterate's L1 norm)
 
     // Sweep first, decide afterwards.  Unless the previous launch ended the solve, this launch
     // performs s_spec = min(S, itmax - (k_prev + sd)) sweeps from the previous output, which the
@@ -295,8 +297,8 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         xch[1][wv][1][lane] = v[(RB - 1 + 1) & 1][RB - 1];
         __syncthreads();
     };
-    // sweep s (red half-sweep, then black) of a pass
-    auto one_sweep = [&](const int s) {
+    // sweep s (red half-sweep, then black) of a pass; with_norm: also sum the iterate's norm
+    auto one_sweep = [&](const int s, const bool with_norm) {
         Acc acc[2] = {Acc(0), Acc(0)}, nrm[2] = {Acc(0), Acc(0)};
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
@@ -324,14 +326,14 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
                     cand = GS ? qt : qt + omb * old;
                     v[q][r] = cand;
                     acc[q] = __fma_rn(w[r], fabs(cand - old), acc[q]);
-                    if (kNorm) nrm[q] = __fma_rn(w[r], fabs(cand), nrm[q]);
+                    if (kNorm && with_norm) nrm[q] = __fma_rn(w[r], fabs(cand), nrm[q]);
                 } else {
                     const T x = __fmaf_rn(c1, up + dn, __fmaf_rn(c2, own + far, fc[q][r]));
                     const T y = be[q][r] * x;
                     cand = GS ? y : __fmaf_rn(omb, old, y);
                     v[q][r] = cand;
                     acc[q] = __fmaf_rn(w[r], fabsf(cand - old), acc[q]);
-                    nrm[q] = __fmaf_rn(w[r], fabsf(cand), nrm[q]);
+                    if (with_norm) nrm[q] = __fmaf_rn(w[r], fabsf(cand), nrm[q]);
                 }
             }
             xch[c][wv][0][lane] = v[c & 1][0];
@@ -339,14 +341,15 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
             __syncthreads();
         }
         // per-lane sums of this sweep, owned columns only; reduced once after the last sweep
-        lred[s][0][tid] = (double)((ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0)));
-        if (kNorm)
-            lred[s][1][tid] = (double)((ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0)));
+        lred[s][tid] = (double)((ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0)));
+        if (kNorm && with_norm)
+            lnrm[tid] = (double)((ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0)));
     };
-    // a pass of ns sweeps; the rollback pass (rare) and a budget-limited last launch take the loop
+    // a pass of ns sweeps; the rollback pass (rare) and a budget-limited last launch take the
+    // loop, which sums the norm in every sweep (the last one's stays)
     auto sweeps = [&](const int ns) {
         sweeps_begin();
-        for (int s = 0; s < ns; ++s) one_sweep(s);
+        for (int s = 0; s < ns; ++s) one_sweep(s, kNorm);
     };
     int mode = 0;          // 0: sweep on, 1: stop (terminal), 2: redo s_run sweeps (rollback)
     int s_run = s_spec;
@@ -357,7 +360,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     if (kUnroll && s_spec == S) {
         sweeps_begin();
 #pragma unroll
-        for (int s = 0; s < S; ++s) one_sweep(s);
+        for (int s = 0; s < S; ++s) one_sweep(s, s == S - 1);
     } else {
         sweeps(s_spec);
     }
@@ -369,53 +372,44 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     // All S slots are reduced (only the first sd are read): no branch separates the chains.
     const int nwp = min(NW, (ntiles + 63) >> 6);  // waves holding partials
     if (sd > 0 && wv < nwp) {
-        double t[S * nq];
-#pragma unroll
-        for (int s = 0; s < S; ++s)
-#pragma unroll
-            for (int q = 0; q < nq; ++q) t[s * nq + q] = pv[s][q];
-        wave_sum_n(t);
+        wave_sum_n(pv);
         if (lane == 0) {
 #pragma unroll
-            for (int s = 0; s < S; ++s)
-#pragma unroll
-                for (int q = 0; q < nq; ++q) lw[wv][s][q] = t[s * nq + q];
+            for (int i = 0; i < nq; ++i) lw[wv][i] = pv[i];
         }
     }
     __syncthreads();  // this launch's lane sums (lred) and the per-wave totals (lw) are complete
 
-    // this launch's per-tile partials (the sweeps' per-lane sums in lred): wave w reduces
-    // (sweep, quantity) rows, in a fixed order.  Written whatever the decision below: a launch
-    // that ends the solve leaves partials nobody reads.
-    for (int row = wv; row < nq * s_spec; row += NW) {
-        const int s = row / nq, q = row - s * nq;
+    // this launch's per-tile partials (the sweeps' per-lane sums in lred, the norm in lnrm):
+    // wave w reduces rows, in a fixed order.  Written whatever the decision below: a launch that
+    // ends the solve leaves partials nobody reads.
+    const int nrows = s_spec > 0 ? s_spec + (kNorm ? 1 : 0) : 0;
+    for (int row = wv; row < nrows; row += NW) {
+        const double* col = row < s_spec ? lred[row] : lnrm;
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < NW; ++j) acc += lred[s][q][lane + 64 * j];
+        for (int j = 0; j < NW; ++j) acc += col[lane + 64 * j];
         acc = wave_sum(acc);
-        if (lane == 0) a.partial[(L & 1) * pstride + ((size_t)s * ntiles + tile) * 2 + q] = acc;
+        const size_t slot = row < s_spec ? ((size_t)row * ntiles + tile) * 2 : (size_t)tile * 2 + 1;
+        if (lane == 0) a.partial[(L & 1) * pstride + slot] = acc;
     }
 
-    // ---- decision on the previous launch (src/poisson.c:272-278): lane s*nq+q of every wave sums
-    //      the per-wave totals of (s, q) in wave order, the same in every workgroup, so all reach
+    // ---- decision on the previous launch (src/poisson.c:272-278): lane i of every wave sums the
+    //      per-wave totals of quantity i in wave order, the same in every workgroup, so all reach
     //      the same decision; the totals are then broadcast to the whole wave -------------------
-    double etot[S][2];
+    double etot[S];      // the previous launch's L1 update of sweep s
+    double ntot = 0.0;   // and its iterate's L1 norm after its last sweep (relative rule)
     {
         double e = 0.0;
-        if (sd > 0 && lane < nq * sd) {
-            const int s = lane / nq, q = lane - s * nq;
-            for (int j = 0; j < nwp; ++j) e += lw[j][s][q];
+        if (sd > 0 && (lane < sd || (kNorm && lane == nq - 1))) {
+            for (int j = 0; j < nwp; ++j) e += lw[j][lane];
         }
         const int lo = __double2loint(e), hi = __double2hiint(e);
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                etot[s][q] = q < nq ? __hiloint2double(__builtin_amdgcn_readlane(hi, s * nq + q),
-                                                       __builtin_amdgcn_readlane(lo, s * nq + q))
-                                    : 0.0;
-            }
-        }
+        for (int s = 0; s < S; ++s)
+            etot[s] = __hiloint2double(__builtin_amdgcn_readlane(hi, s), __builtin_amdgcn_readlane(lo, s));
+        if (kNorm)
+            ntot = __hiloint2double(__builtin_amdgcn_readlane(hi, nq - 1), __builtin_amdgcn_readlane(lo, nq - 1));
     }
     SorCtl next;
     next.k = k_prev;
@@ -429,23 +423,21 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     next.n_last = prev->n_last;
     if (sd > 0) {
         // the first sweep s < sd with e < tol (converged) or, failing that, below rtol times the
-        // iterate's norm (fold); static indices keep etot in registers
+        // iterate's norm after the launch (fold); static indices keep etot in registers
         int s_conv = -1, s_fold = -1;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             if (s < sd && s_conv < 0 && s_fold < 0) {
-                if (etot[s][0] < tol) s_conv = s;
-                else if (rtol > 0.0 && etot[s][0] < rtol * etot[s][1]) s_fold = s;
+                if (etot[s] < tol) s_conv = s;
+                else if (rtol > 0.0 && etot[s] < rtol * ntot) s_fold = s;
             }
         }
         const int s_rep = (s_conv >= 0 && s_conv < sd - 1) ? s_conv : sd - 1;  // reported sweep
-        double e_rep = 0.0, n_rep = 0.0;
+        double e_rep = 0.0;
+        const double n_rep = ntot;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if (s == s_rep) {
-                e_rep = etot[s][0];
-                n_rep = etot[s][1];
-            }
+            if (s == s_rep) e_rep = etot[s];
         }
         if (s_conv >= 0 && s_conv < sd - 1) {
             // converged inside the previous launch: redo s_conv+1 sweeps from its
