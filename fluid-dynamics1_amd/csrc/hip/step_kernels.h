@@ -71,11 +71,24 @@ __device__ inline void wall_vorticity_at(double* w, const double* u, const doubl
     const double* c1x = s.d1x + (size_t)j * kStencilWidth;
     const double* c1y = s.d1y + (size_t)i * kStencilWidth;
     const int sj = s.start[j], si = s.start[i];
+    // the same values as bc_u / bc_v, chosen among copies of the wall velocities (selecting the
+    // struct's fields by reference in this loop made the compiler spill the struct to scratch);
+    // every stencil point lies inside the grid, so its interior value is always loadable
+    const double u1 = b.u1, u2 = b.u2, u3 = b.u3, u4 = b.u4;
+    const double v1 = b.v1, v2 = b.v2, v3 = b.v3, v4 = b.v4;
+    auto pick = [n](double in, int pi, int pj, double w1, double w2, double w3, double w4) {
+        double r = pi == n - 1 ? w4 : in;
+        r = pi == 0 ? w3 : r;
+        r = pj == n - 1 ? w2 : r;
+        return pj == 0 ? w1 : r;
+    };
     double dvdx = 0.0, dudy = 0.0;
 #pragma unroll
-    for (int m = 0; m < kStencilWidth; ++m) dvdx += c1x[m] * bc_v(v, i, sj + m, n, ld, b);
+    for (int m = 0; m < kStencilWidth; ++m)
+        dvdx += c1x[m] * pick(v[(size_t)i * ld + sj + m], i, sj + m, v1, v2, v3, v4);
 #pragma unroll
-    for (int m = 0; m < kStencilWidth; ++m) dudy += c1y[m] * bc_u(u, si + m, j, n, ld, b);
+    for (int m = 0; m < kStencilWidth; ++m)
+        dudy += c1y[m] * pick(u[(size_t)(si + m) * ld + j], si + m, j, u1, u2, u3, u4);
     w[(size_t)i * ld + j] = dvdx - dudy;
 }
 
