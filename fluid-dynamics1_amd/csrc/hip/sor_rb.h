@@ -134,11 +134,7 @@ __device__ inline float dpp_shift(float x) {
 constexpr int kDppFromNext = 0x130;  // wave_shl:1
 constexpr int kDppFromPrev = 0x138;  // wave_shr:1
 
-// Sum over the 64 lanes of a wave, returned to every lane, in a fixed association (the same in
-// every workgroup): pairs and quads (quad_perm [1,0,3,2] and [2,3,0,1]), the four quads of each
-// 16-lane row (row_shr 4 and 8: lane 15 of a row holds the row's sum), then the rows
-// (row_bcast 15: lane 63 gets rows 3 + 2, lane 31 rows 1 + 0; row_bcast 31: lane 63 gets all
-// four); lane 63's value is broadcast.  Cross-lane moves only, no LDS round trips.
+// One step of a wave reduction of N values: x[i] += the lane value that CTRL selects.
 template <int CTRL, int N>
 __device__ inline void dpp_accumulate(double (&x)[N]) {
     double y[N];
@@ -147,8 +143,13 @@ __device__ inline void dpp_accumulate(double (&x)[N]) {
 #pragma unroll
     for (int i = 0; i < N; ++i) x[i] += y[i];
 }
-// N independent sums, each step applied to all of them before the next, so that their chains
-// interleave (no idle cycles between a value's update and its next cross-lane read)
+// Sums over the 64 lanes of a wave of N independent values, returned to every lane, in a fixed
+// association (the same in every workgroup): pairs and quads (quad_perm [1,0,3,2] and
+// [2,3,0,1]), the four quads of each 16-lane row (row_shr 4 and 8: lane 15 of a row holds the
+// row's sum), then the rows (row_bcast 15: lane 63 gets rows 3 + 2, lane 31 rows 1 + 0;
+// row_bcast 31: lane 63 gets all four); lane 63's value is broadcast.  Cross-lane moves only, no
+// LDS round trips; each step is applied to all N values before the next, so that their chains
+// interleave (no idle cycles between a value's update and its next cross-lane read).
 template <int N>
 __device__ inline void wave_sum_n(double (&x)[N]) {
     dpp_accumulate<0xb1>(x);
