@@ -230,9 +230,9 @@ def main():
                          "calibration copy (for counter scaling); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the single cavity would "
-              "shard by row strips with halo exchange; see DESIGN.md). Run with --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the single cavity would shard "
+              "by row strips with a halo exchange every few sweeps, the batch by whole cavities). "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import cnv
