@@ -43,7 +43,7 @@ struct cnv_ctx {
     // launch shape
     cnv::TileShape shape{5, 16, 4};
     int tiles_i = 0, tiles_j = 0;
-    int launches_per_poll = 32;
+    int launches_per_poll = 64;
     int k_prev = 0;       // sweeps of the previous solve (queue-ahead prediction)
     cnv::WallBC bc{};
     // captured step (use_graph, FP64): pre-solve launches per w parity, a chunk of
