@@ -233,7 +233,7 @@ struct cnv_batch {
     hipStream_t stream = nullptr;
     std::vector<cnv_ctx*> ctx;
     TileShape shape{5, 16, 4};
-    int launches_per_poll = 16;
+    int launches_per_poll = 4;
     int wcur = 0;
     int fin_slot = 0;
     int k_prev = 0;  // most sweeps any cavity needed in the previous solve
@@ -581,7 +581,10 @@ int cnv_batch_create(const cnv_params* items, int count, cnv_batch** out) {
     b->device = p0.device;
     b->rtol = p0.inner_rtol;
     b->max_outer = p0.max_outer > 0 ? p0.max_outer : 50;
-    b->launches_per_poll = p0.launches_per_poll > 0 ? p0.launches_per_poll : 16;
+    // 4 per poll: a batch launch covers every cavity's tiles (tens of microseconds), and each
+    // solve's end wastes up to a chunk of them; the 64-cavity workload measured 4% faster per
+    // step (FP32-refined) and 2% (FP64) than with 16
+    b->launches_per_poll = p0.launches_per_poll > 0 ? p0.launches_per_poll : 4;
     auto setup = [&]() -> int {
         for (int i = 0; i < count; ++i) {
             cnv_ctx* c = nullptr;

@@ -56,7 +56,7 @@ typedef struct {
     int tile_waves;        /* wave64s per SOR workgroup (compiled shapes only)               */
     int wave_rows;         /* grid rows held in registers per wave (compiled shapes only)    */
     int launches_per_poll; /* SOR launches queued between host polls of the convergence flag
-                              (0: 64 for a cavity, 16 for a batch) */
+                              (0: 64 for a cavity, 4 for a batch) */
     int diagnostics;       /* 1 = continuity max/min every step (src/main.c:387-408)        */
     int use_graph;         /* FP64: 1 = each step replays captured HIP graphs (the pre-solve
                               launches, chunks of launches_per_poll SOR launches, the post-solve
