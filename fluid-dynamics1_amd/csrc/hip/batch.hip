@@ -637,7 +637,10 @@ int cnv_batch_create(const cnv_params* items, int count, cnv_batch** out) {
         }
         int rc = upload(b, &b->d_items, it);
         if (!rc) rc = build_maps(b);
-        if (!rc) rc = build_sor(b, b->ctx[0]->shape);
+        // the items' launch shape when they name one (cnv_create checked it), else (5, 16, 4):
+        // a one-cavity default follows that cavity's size, a batch's spans its cavities
+        const bool named = p0.sweeps_per_launch > 0 || p0.tile_waves > 0 || p0.wave_rows > 0;
+        if (!rc) rc = build_sor(b, named ? b->ctx[0]->shape : TileShape{5, 16, 4});
         if (!rc) BHIP(b, hipDeviceSynchronize());
         return rc;
     };

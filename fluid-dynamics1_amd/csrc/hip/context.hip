@@ -94,7 +94,20 @@ int cnv_device_name(int device, char* buf, int len) {
     return CNV_OK;
 }
 
-// Default launch shape per sweeps-per-launch (tuned on MI355X at 1024^2; 0 = the default).
+// The default shape for an n x n grid on a device with `cus` compute units: the first of the
+// thin-to-thick 5-sweep tiles whose count fits one workgroup per CU (measured per sweep, FP64:
+// 128^2 and 256^2 best with (5, 4, 6), 512^2 with (5, 8, 4), 1024^2 with (5, 16, 4)); grids
+// beyond take the thickest tile, which has the least halo.
+static TileShape auto_shape(int n, int cus) {
+    const TileShape cand[] = {{5, 4, 6}, {5, 8, 4}, {5, 16, 4}};
+    for (const TileShape& t : cand) {
+        const long long tiles = (long long)((n + tile_rows(t) - 1) / tile_rows(t)) * ((n + tile_cols(t) - 1) / tile_cols(t));
+        if (tiles <= cus) return t;
+    }
+    return cand[2];
+}
+
+// Default launch shape per sweeps-per-launch (tuned on MI355X at 1024^2; 0 = by grid size).
 static TileShape shape_for(int spl) {
     switch (spl) {
         case 0: return TileShape{5, 16, 4};
@@ -257,6 +270,12 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
     c->bc = WallBC{p->u1, p->u2, p->u3, p->u4, p->v1, p->v2, p->v3, p->v4};
     const int spl = p->sweeps_per_launch;
     c->shape = shape_for(spl);
+    if (spl == 0 && p->tile_waves <= 0 && p->wave_rows <= 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || cus <= 0)
+            cus = 256;
+        c->shape = auto_shape(p->n, cus);
+    }
     if (p->tile_waves > 0) c->shape.nw = p->tile_waves;
     if (p->wave_rows > 0) c->shape.rb = p->wave_rows;
     if ((p->sweeps_per_launch && c->shape.s != p->sweeps_per_launch) || !sor_shape_supported(c->shape)) {
@@ -264,9 +283,11 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
         return cnv_fail(nullptr, CNV_ERR_ARG, "unsupported SOR launch shape (sweeps_per_launch %d, waves %d, rows per wave %d)",
                     spl, p->tile_waves, p->wave_rows);
     }
-    // 64 per poll: each poll costs a copy in the stream, each chunk in flight after convergence
-    // up to 64 no-op launches; at 1024^2 64 measured 0.6% faster per step than 32 or 128
+    // captured chunks of 64 launches: each replay costs a graph launch and a poll copy in the
+    // stream, and a solve's end up to two chunks of no-op launches; at 1024^2 64 measured 0.6%
+    // faster per step than 32 or 128.  Eager solves choose their chunk per solve (run_sor).
     c->launches_per_poll = p->launches_per_poll > 0 ? p->launches_per_poll : 64;
+    c->poll_fixed = p->launches_per_poll > 0 ? p->launches_per_poll : 0;
     c->tiles_i = (c->n + tile_rows(c->shape) - 1) / tile_rows(c->shape);
     c->tiles_j = (c->n + tile_cols(c->shape) - 1) / tile_cols(c->shape);
     auto setup = [&]() -> int {
@@ -335,8 +356,10 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) 
         LAUNCH(c, dc_inner_init_launch(c->e32[0], c->ctl, c->n, c->ld, c->p.inner_rtol, c->p.tol, c->p.itmax - total,
                                        c->stream));
         SorCtl snap{};
+        c->k_prev = o < cnv_ctx::kDcPred ? c->k_dc[o] : 0;  // this outer iteration's last length
         int rc = run_sor<float>(c, a, &snap);
         if (rc) return rc;
+        if (o < cnv_ctx::kDcPred) c->k_dc[o] = snap.k;
         LAUNCH(c, dc_correct_launch(c->psi[0], c->e32[0], c->e32[1], c->ctl + c->fin_slot, c->n, c->ld, c->stream));
         total += snap.k;
         *outer = o + 1;

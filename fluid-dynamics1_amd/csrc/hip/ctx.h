@@ -43,8 +43,11 @@ struct cnv_ctx {
     // launch shape
     cnv::TileShape shape{5, 16, 4};
     int tiles_i = 0, tiles_j = 0;
-    int launches_per_poll = 64;
-    int k_prev = 0;       // sweeps of the previous solve (queue-ahead prediction)
+    int launches_per_poll = 64;  // SOR launches per captured chunk (graph path)
+    int poll_fixed = 0;   // launches per poll as requested; 0: chosen per solve (eager path)
+    int k_prev = 0;       // sweeps the next solve is predicted to take (the previous one's)
+    static constexpr int kDcPred = 64;
+    int k_dc[kDcPred] = {};  // FP32_DC: sweeps of the inner solve of each outer iteration
     cnv::WallBC bc{};
     // captured step (use_graph, FP64): pre-solve launches per w parity, a chunk of
     // launches_per_poll SOR launches per source plane, post-solve launches
@@ -125,17 +128,25 @@ inline bool sor_done(int status) { return status != cnv::SOR_RUNNING; }
 // Queue SOR launches (launch index L = 0, 1, ...) until the state slot written by a queued
 // launch reports a terminal status.  Keeps one chunk queued behind the one being waited on, so
 // the GPU never idles on the host; launches queued after the end are no-ops.
+// With a prediction of P launches (k_prev), the first chunk is P - M launches and the rest come
+// M at a time.  Each poll costs a copy in the stream (about 5.6 us), and the end of a solve wastes
+// M to 2M no-op launches (about 3.8 us each at 1024^2); with a prediction off by about 2% of P
+// the cost is smallest near M = sqrt(P / 50), kept within [4, 32].  Without a prediction M = 16.
+inline int poll_chunk(const cnv_ctx* c, int predicted_launches) {
+    if (c->poll_fixed > 0) return c->poll_fixed;
+    if (predicted_launches <= 0) return 16;
+    const int m = (int)std::lround(std::sqrt(predicted_launches / 50.0));
+    return m < 4 ? 4 : (m > 32 ? 32 : m);
+}
 template <typename T>
 inline int run_sor(cnv_ctx* c, const cnv::SorArgs<T>& a, cnv::SorCtl* final_ctl) {
     const bool gs = (c->p.poisson_type == 1);
     const int S = c->shape.s;
-    const int M = c->launches_per_poll;
+    const int P = c->k_prev > 0 ? c->k_prev / S : 0;  // predicted launches
+    const int M = poll_chunk(c, P);
     const long long cap = (long long)c->p.itmax + 4LL * M + 8;  // working launches <= itmax/S + 2
     int first = M;
-    if (c->k_prev > 0) {
-        const int predicted = c->k_prev / S - M;
-        if (predicted > first) first = predicted;
-    }
+    if (P - M > first) first = P - M;
     long long queued = 0;
     int slot = 0;
     int pending[kSlots];

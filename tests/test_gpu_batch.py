@@ -36,10 +36,13 @@ def run_alone(p, steps):
         return st, cav.get_state()
 
 
+@pytest.mark.parametrize("shape", [(5, 16, 4), (5, 4, 6)], ids=["s5_16x4", "s5_4x6"])
 @pytest.mark.parametrize("precision,steps", [(cnv.PREC_FP64, 3), (cnv.PREC_FP32_DC, 2)])
-def test_batch_equals_single(precision, steps):
-    ps = mix_params(precision)
-    print("batch of", [p.n for p in ps], "precision", precision, flush=True)
+def test_batch_equals_single(precision, steps, shape):
+    # the same launch shape on both sides, named explicitly (a one-cavity default follows the
+    # cavity's size): the reported L1 update is summed per tile
+    ps = mix_params(precision, sweeps_per_launch=shape[0], tile_waves=shape[1], wave_rows=shape[2])
+    print("batch of", [p.n for p in ps], "precision", precision, "shape", shape, flush=True)
     with cnv.Batch(ps) as b:
         bst = b.step(steps)
         bstate = [b.get_state(i) for i in range(len(ps))]

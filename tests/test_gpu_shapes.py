@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 # S, waves, rows per wave: the shapes the library compiles
 SHAPES = [(1, 4, 10), (2, 4, 10), (4, 8, 8), (6, 8, 8), (8, 8, 10), (4, 16, 4), (5, 16, 4), (6, 16, 4),
-          (5, 12, 6)]
+          (5, 12, 6), (5, 8, 4), (4, 8, 4), (5, 4, 6)]
 WALLS = dict(ui=0.1, vi=-0.05, u=(0.2, -0.1, 0.05, 1.2), v=(-0.3, 0.1, 0.25, -0.15))
 
 
@@ -62,6 +62,14 @@ def test_every_shape_refined_form():
         for name in ("psi", "w", "u", "v"):
             err = np.max(np.abs(s32[name] - s64[name])) / np.max(np.abs(s64[name]))
             assert err <= 1e-6, (sh, name, err)
+
+
+def test_default_shape_follows_the_grid():
+    # thin tiles on small grids (the GPU is otherwise idle), thick ones from 1024^2 on; the
+    # results do not depend on the shape (bit-exact above), only the speed
+    for n, want in ((64, 5), (256, 5), (1024, 5)):
+        with cnv.Cavity(cnv.default_params(n)) as cav:
+            assert cav.sweeps_per_launch == want
 
 
 def test_uncompiled_shape_refused():
