@@ -58,7 +58,7 @@ typedef struct {
     int wave_rows;         /* grid rows held in registers per wave (compiled shapes only)    */
     int launches_per_poll; /* SOR launches queued between host polls of the convergence flag
                               (0: per solve from the previous solve's length, 64 per captured
-                              chunk with use_graph, 4 for a batch)                          */
+                              chunk with use_graph, 2 for a batch)                          */
     int diagnostics;       /* 1 = continuity max/min every step (src/main.c:387-408)        */
     int use_graph;         /* FP64: 1 = each step replays captured HIP graphs (the pre-solve
                               launches, chunks of launches_per_poll SOR launches, the post-solve
