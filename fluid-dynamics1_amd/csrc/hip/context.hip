@@ -94,17 +94,19 @@ int cnv_device_name(int device, char* buf, int len) {
     return CNV_OK;
 }
 
-// The default shape for an n x n grid on a device with `cus` compute units: the first of the
-// thin-to-thick 5-sweep tiles whose count fits one workgroup per CU (measured per sweep, FP64:
-// 128^2 and 256^2 best with (5, 4, 6), 512^2 with (5, 8, 4), 1024^2 with (5, 16, 4)); grids
-// beyond take the thickest tile, which has the least halo.
+// The default shape for an n x n grid on a device with `cus` compute units: the first of these
+// tiles whose count fits one workgroup per CU -- thin tiles first (on a small grid the GPU is
+// otherwise idle), then six sweeps per launch on the 16-wave tile, then five.  Measured per
+// sweep, FP64: 128^2 and 256^2 best with (5, 4, 6), 512^2 with (5, 8, 4), 768^2 with (6, 16, 4)
+// (2.21 us against 2.42 us with (5, 16, 4)), 1024^2 with (5, 16, 4) ((6, 16, 4) needs 260 tiles
+// there).  Grids beyond take (5, 16, 4), the thickest tile, which has the least halo.
 static TileShape auto_shape(int n, int cus) {
-    const TileShape cand[] = {{5, 4, 6}, {5, 8, 4}, {5, 16, 4}};
+    const TileShape cand[] = {{5, 4, 6}, {5, 8, 4}, {6, 16, 4}, {5, 16, 4}};
     for (const TileShape& t : cand) {
         const long long tiles = (long long)((n + tile_rows(t) - 1) / tile_rows(t)) * ((n + tile_cols(t) - 1) / tile_cols(t));
         if (tiles <= cus) return t;
     }
-    return cand[2];
+    return cand[3];
 }
 
 // Default launch shape per sweeps-per-launch (tuned on MI355X at 1024^2; 0 = by grid size).

@@ -569,11 +569,11 @@ inline int sor_launch_shape(const SorArgs<T>& a, bool gs, hipStream_t st) {
 
 // The launch shapes compiled in, TileShape {s, nw, rb}: S sweeps per launch, nw waves of rb rows;
 // the tile owns (nw*rb - 4s) rows x (128 - 4s) columns.  The default is the first of (5, 4, 6),
-// (5, 8, 4), (5, 16, 4) whose tiles fit the device's CUs one per CU: on a small grid the GPU is
-// otherwise idle, so thin tiles (more halo, shorter workgroups) win -- per sweep at 256^2 FP64
-// 1.50 us with (5, 4, 6) (192 workgroups of 4 waves) against 2.21 us with (5, 16, 4) (18
-// workgroups); at 1024^2 (5, 16, 4) gives 240 workgroups of 16 waves (four per SIMD), 1.72x halo
-// redundancy.  The others serve the batched autotuner and the sweep tests.
+// (5, 8, 4), (6, 16, 4), (5, 16, 4) whose tiles fit the device's CUs one per CU: on a small
+// grid the GPU is otherwise idle, so thin tiles (more halo, shorter workgroups) win -- per sweep
+// at 256^2 FP64 1.50 us with (5, 4, 6) (192 workgroups of 4 waves) against 2.21 us with
+// (5, 16, 4) (18 workgroups); at 1024^2 (5, 16, 4) gives 240 workgroups of 16 waves (four per
+// SIMD), 1.72x halo redundancy.  The others serve the batched autotuner and the sweep tests.
 #define CNV_SOR_SHAPES(X) \
     X(1, 4, 10)           \
     X(2, 4, 10)           \

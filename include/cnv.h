@@ -52,8 +52,8 @@ typedef struct {
     double inner_rtol;     /* inner FP32 solve: fold once its L1 update < inner_rtol * L1 norm of the inner iterate */
     int max_outer;         /* defect-correction iterations allowed per solve                */
     /* tuning (0 = automatic) */
-    int sweeps_per_launch; /* SOR sweeps fused per kernel launch (compiled: 1, 2, 4, 5, 6, 8; 0 = 5
-                              with the tile shape chosen by grid size)                        */
+    int sweeps_per_launch; /* SOR sweeps fused per kernel launch (compiled: 1, 2, 4, 5, 6, 8; 0 =
+                              5 or 6 with the tile shape, both chosen by grid size)          */
     int tile_waves;        /* wave64s per SOR workgroup (compiled shapes only)               */
     int wave_rows;         /* grid rows held in registers per wave (compiled shapes only)    */
     int launches_per_poll; /* SOR launches queued between host polls of the convergence flag

@@ -67,7 +67,7 @@ def test_every_shape_refined_form():
 def test_default_shape_follows_the_grid():
     # thin tiles on small grids (the GPU is otherwise idle), thick ones from 1024^2 on; the
     # results do not depend on the shape (bit-exact above), only the speed
-    for n, want in ((64, 5), (256, 5), (1024, 5)):
+    for n, want in ((64, 5), (256, 5), (768, 6), (1024, 5)):
         with cnv.Cavity(cnv.default_params(n)) as cav:
             assert cav.sweeps_per_launch == want
 
