@@ -67,7 +67,7 @@ def read_traffic(name):
     return None
 
 
-def cpu_baseline(sample_seconds=12.0):
+def cpu_baseline(sample_seconds=25.0):
     """Oracle at the same workload on the host cores: the first time step's transport plus a
     bounded number of red-black SOR sweeps, timed; scaled to cell-steps/s with the GPU run's
     mean sweeps per step (identical arithmetic gives identical sweep counts)."""
@@ -87,7 +87,7 @@ def cpu_baseline(sample_seconds=12.0):
     t0 = time.perf_counter()
     orc.poisson(p, f)
     per = (time.perf_counter() - t0) / 4
-    sweeps = int(max(8, min(4000, sample_seconds / max(per, 1e-9))))
+    sweeps = int(max(8, min(40000, sample_seconds / max(per, 1e-9))))
     p.itmax = sweeps
     t0 = time.perf_counter()
     orc.poisson(p, f)
