@@ -51,7 +51,8 @@ static void help_config(void)
     printf("            u3/v3 row 0, u4/v4 last row (u4 = 1.0 is the moving lid)\n");
     printf("Added keys: sor_omega (SOR relaxation; default 0.5*(2/(1+sin(PI/(nx+1)))+2/(1+sin(PI/(ny+1))))),\n");
     printf("            nsteps (default (int)(tf/dt-1)+1), precision (fp64 | fp32dc), inner_rtol (1e-5),\n");
-    printf("            sweeps_per_launch (0 = tuned default), diagnostics (1), device (0)\n");
+    printf("            inner_beta (fp32dc inner relaxation: 0 = sor_omega, or \"reference\"),\n");
+    printf("            sweeps_per_launch (0 = tuned default), diagnostics (1), device (0), use_graph (0)\n");
 }
 
 static mtrx to_mtrx(const double *a, int n)
@@ -130,6 +131,7 @@ int main(int argc, char *argv[])
     p.u1 = cfg.u1; p.u2 = cfg.u2; p.u3 = cfg.u3; p.u4 = cfg.u4;
     p.v1 = cfg.v1; p.v2 = cfg.v2; p.v3 = cfg.v3; p.v4 = cfg.v4;
     p.inner_rtol = cx.inner_rtol;
+    p.inner_beta = cx.inner_beta < 0 ? cnv_reference_beta(cfg.nx, cfg.ny) : cx.inner_beta;
     p.sweeps_per_launch = cx.sweeps_per_launch;
     p.diagnostics = cx.diagnostics;
     p.use_graph = cx.use_graph;

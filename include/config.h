@@ -41,6 +41,8 @@ typedef struct {
     int diagnostics;        /* diagnostics: 1 = continuity max/min every step                  */
     int device;             /* device: HIP device ordinal                                        */
     int use_graph;          /* use_graph: 1 = replay each FP64 step from captured HIP graphs    */
+    double inner_beta;      /* inner_beta: relaxation of the inner FP32 solves of fp32dc (0 =
+                               sor_omega; "reference" = the reference's formula for nx, ny)     */
     int has_sor_omega;
 } ConfigExt;
 

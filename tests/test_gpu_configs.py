@@ -1,0 +1,48 @@
+"""The workload configuration files under configs/ (the reference's key = value format plus this
+build's keys) drive the cnavier driver: two steps of each, shortened with nsteps, log the
+reference's per-step lines; inner_beta = reference relaxes the refined form's inner solves."""
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "fluid-dynamics1_amd", "cnavier")
+CONFIGS = sorted(glob.glob(os.path.join(ROOT, "configs", "*.txt")))
+SOLVED = re.compile(r"solved with (\d+) iterations")
+
+pytestmark = pytest.mark.gpu
+
+
+def run(tmp_path, text, name):
+    (tmp_path / "cfg.txt").write_text(text)
+    r = subprocess.run([DRIVER, "cfg.txt", name], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return (tmp_path / "output" / "logs" / (name + ".txt")).read_text()
+
+
+def test_config_files_present():
+    names = [os.path.basename(p) for p in CONFIGS]
+    assert names == ["config0_cpu_128.txt", "config1_gpu_1024.txt", "config2_refined_1024.txt"]
+
+
+@pytest.mark.parametrize("path", CONFIGS, ids=lambda p: os.path.basename(p)[:-4])
+def test_config_runs_two_steps(tmp_path, path):
+    print("config", os.path.basename(path), flush=True)
+    text = open(path).read() + "\nnsteps = 2\noutput_interval = 1000000\n"
+    log = run(tmp_path, text, "cfg")
+    ks = [int(k) for k in SOLVED.findall(log)]
+    assert len(ks) == 2 and all(k > 0 for k in ks), log[-1500:]
+
+
+def test_inner_beta_reference_key(tmp_path):
+    base = open(os.path.join(ROOT, "configs", "config2_refined_1024.txt")).read()
+    base = base.replace("nx = 1024", "nx = 256").replace("ny = 1024", "ny = 256")
+    base += "\nnsteps = 2\noutput_interval = 1000000\npoisson_tol = 1e-9\n"
+    plain = [int(k) for k in SOLVED.findall(run(tmp_path, base, "plain"))]
+    relaxed = [int(k) for k in SOLVED.findall(run(tmp_path, base + "inner_beta = reference\n", "ref"))]
+    print("inner sweeps, inner_beta = sor_omega:", plain, " reference:", relaxed, flush=True)
+    assert len(plain) == len(relaxed) == 2
+    assert sum(relaxed) < 0.5 * sum(plain)
