@@ -37,12 +37,12 @@ BYTES_PER_CELL_SWEEP_FP32 = 12
 C_CELLS = N * N
 
 
-def workload_params(cnv, precision, args):
+def workload_params(cnv, precision, args, **kw):
     # FP64: each step replays captured HIP graphs unless --eager
     return cnv.default_params(N, Re=RE, dt=DT, order=6, beta=OMEGA, tol=TOL, itmax=50_000_000,
                               precision=precision, sweeps_per_launch=args.spl, tile_waves=args.tile_waves,
                               wave_rows=args.wave_rows, diagnostics=1,
-                              use_graph=0 if args.eager else int(precision == cnv.PREC_FP64))
+                              use_graph=0 if args.eager else int(precision == cnv.PREC_FP64), **kw)
 
 
 def timed_steps(cav, k, tag):
@@ -220,6 +220,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--eager", action="store_true", help="FP64 steps as eager launches instead of graph replays")
     ap.add_argument("--no-dc", action="store_true", help="skip the FP32-refined leg")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start FP64 leg")
     ap.add_argument("--sor-launches", type=int, default=400)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batched leg")
     ap.add_argument("--batch-steps", type=int, default=3)
@@ -266,6 +267,23 @@ def main():
     sor_ms = cav.time_sor(args.sor_launches)
     cav.close()
     achieved = spl * BYTES_PER_CELL_SWEEP * C / (sor_ms / 1e3)
+
+    # ---- the same steps with the opt-in warm start (each solve from the previous psi) ---------
+    warm = None
+    if not args.no_warm:
+        with cnv.Cavity(workload_params(cnv, cnv.PREC_FP64, args, warm_start=1)) as cw:
+            if args.warmup:
+                timed_steps(cw, args.warmup, "fp64 warm-start warmup")
+            stw, msw = timed_steps(cw, args.steps, "fp64 warm-start")
+            sw = cw.get_state()
+        warm = {"ms_per_step": msw / args.steps, "value": C * args.steps / (msw / 1e3) / 1e6,
+                "speedup_vs_cold": ms / msw, "sweeps_per_step": [s.sweeps for s in stw],
+                "err_linf_rel_vs_cold": {f: float(np.max(np.abs(sw[f] - state64[f])) / np.max(np.abs(state64[f])))
+                                         for f in ("psi", "w", "u", "v")},
+                "note": "opt-in: each Poisson solve starts from the previous step's psi (the reference "
+                        "starts from 0, src/poisson.c:229); not the metric's configuration"}
+        print("bench: fp64 warm start %.3f ms/step, sweeps/step %s" % (warm["ms_per_step"], warm["sweeps_per_step"]),
+              file=sys.stderr, flush=True)
 
     # ---- FP32 + FP64 defect correction ------------------------------------------------------
     dc = None
@@ -321,6 +339,7 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "sor_rb_kernel<double,...,S=%d>" % spl, "launch_ms": sor_ms},
         "cpu_baseline": cpu,
+        "fp64_warm_start": warm,
         "fp32_refined": dc,
         "batched": batched,
         "adjoint": adjoint,

@@ -581,6 +581,9 @@ int cnv_batch_create(const cnv_params* items, int count, cnv_batch** out) {
             return bfail(nullptr, CNV_ERR_ARG,
                          "batch item %d: precision, order, poisson_type, device and launch shape must match item 0", i);
     }
+    for (int i = 0; i < count; ++i)
+        if (items[i].warm_start)
+            return bfail(nullptr, CNV_ERR_ARG, "batch item %d: warm_start is for one-cavity contexts only", i);
     cnv_batch* b = new cnv_batch();
     b->count = count;
     b->dc = p0.precision == CNV_PREC_FP32_DC;

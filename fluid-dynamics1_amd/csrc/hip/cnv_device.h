@@ -114,8 +114,11 @@ int transport_fp64_launch(const double* w_in, double* w_out, const double* u, co
 int transport_fp32_launch(const double* w_in, double* w_out, const double* u, const double* v,
                           int n, int ld, const Stencils& s, const WallBC& bc, double rRe,
                           double dt, hipStream_t st);
+// zero = 0 keeps psi0 (warm start) and only resets the state slots and the diagnostic
 int poisson_init_launch(double* psi0, SorCtl* ctl, Diag* diag, int n, int ld, double tol,
-                        int itmax, hipStream_t st);
+                        int itmax, hipStream_t st, int zero = 1);
+// warm start: psi0 <- psi[fin->parity] (the previous solve's final iterate)
+int warm_start_launch(double* psi0, const double* psi1, const SorCtl* fin, int n, int ld, hipStream_t st);
 int velocity_launch(const double* psi0, const double* psi1, const SorCtl* fin, double* u,
                     double* v, int n, int ld, const Stencils& s, hipStream_t st);
 int continuity_launch(const double* u, const double* v, Diag* diag, int n, int ld,

@@ -335,7 +335,10 @@ int cnv_sweeps_per_launch(const cnv_ctx* c) { return c->shape.s; }
 
 // ---- SOR driver ---------------------------------------------------------------------------------
 static int poisson_fp64(cnv_ctx* c, const double* f_src, int negate, SorCtl* fin) {
-    LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, c->p.tol, c->p.itmax, c->stream));
+    if (c->p.warm_start)
+        LAUNCH(c, warm_start_launch(c->psi[0], c->psi[1], c->ctl + c->fin_slot, c->n, c->ld, c->stream));
+    LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, c->p.tol, c->p.itmax, c->stream,
+                                  !c->p.warm_start));
     SorArgs<double> a = sor_args<double>(c, f_src, c->psi[0], c->psi[1], negate);
     int rc = run_sor<double>(c, a, fin);
     if (rc) return rc;
@@ -345,7 +348,9 @@ static int poisson_fp64(cnv_ctx* c, const double* f_src, int negate, SorCtl* fin
 
 // defect correction: psi[0] accumulates in FP64, e32 ping-pong holds the inner FP32 iterates
 static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) {
-    LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, c->p.tol, c->p.itmax, c->stream));
+    // psi[0] is the FP64 accumulator: a warm start keeps the previous step's final iterate there
+    LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, c->p.tol, c->p.itmax, c->stream,
+                                  !c->p.warm_start));
     const double dx = c->p.dx, dy = c->p.dy;
     const double c1 = dy * dy, c2 = dx * dx, c3 = dx * dx * dy * dy, B = 2 * (dx * dx + dy * dy);
     SorArgs<float> a = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
@@ -399,7 +404,10 @@ static int body_pre(cnv_ctx* c, int wc) {
     LAUNCH(c, wall_vorticity_launch(c->w[wc], c->u, c->v, n, ld, c->st, c->bc, c->stream));
     LAUNCH(c, transport_fp64_launch(c->w[wc], c->w[wc ^ 1], c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re, c->p.dt,
                                     c->stream));
-    LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, n, ld, c->p.tol, c->p.itmax, c->stream));
+    // warm start: whole chunks end on slot 1, which holds the previous solve's final state
+    if (c->p.warm_start) LAUNCH(c, warm_start_launch(c->psi[0], c->psi[1], c->ctl + 1, n, ld, c->stream));
+    LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, n, ld, c->p.tol, c->p.itmax, c->stream,
+                                  !c->p.warm_start));
     return CNV_OK;
 }
 // M SOR launches, L = 0..M-1, on the source w[src] (f = -w)

@@ -32,11 +32,22 @@ __global__ void __launch_bounds__(kBlock) transport_fp64_kernel(const double* __
 
 // ---- start of a Poisson solve: psi = 0 (src/poisson.c:229-230), reset the control block ----
 __global__ void __launch_bounds__(kBlock) poisson_init_kernel(double* psi0, SorCtl* ctl, Diag* diag,
-                                                             long long count, double tol, int itmax) {
+                                                             long long count, double tol, int itmax,
+                                                             int zero) {
     const long long stride = (long long)gridDim.x * kBlock;
-    for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) psi0[q] = 0.0;
+    if (zero)
+        for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) psi0[q] = 0.0;
     if (blockIdx.x == 0 && threadIdx.x < 2) ctl_reset(ctl + threadIdx.x, tol, 0.0, itmax);
     if (blockIdx.x == 0 && threadIdx.x == 0 && diag) diag_reset(diag);
+}
+
+// warm start (opt-in): the previous solve's final iterate, psi[fin->parity], becomes psi0, the
+// first iterate of the next solve (read before the slots are reset by the init kernel)
+__global__ void __launch_bounds__(kBlock) warm_start_kernel(double* psi0, const double* psi1,
+                                                           const SorCtl* fin, long long count) {
+    if (!fin->parity) return;  // already in psi0
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) psi0[q] = psi1[q];
 }
 
 // ---- velocities u = DY psi, v = -(DX psi) at every point (src/main.c:366-383) --------------
@@ -116,11 +127,20 @@ int transport_fp64_launch(const double* w_in, double* w_out, const double* u, co
 }
 
 int poisson_init_launch(double* psi0, SorCtl* ctl, Diag* diag, int n, int ld, double tol, int itmax,
-                        hipStream_t st) {
+                        hipStream_t st, int zero) {
     const long long count = (long long)n * ld;
     unsigned nb = blocks_for(count);
     if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(poisson_init_kernel, dim3(nb), dim3(kBlock), 0, st, psi0, ctl, diag, count, tol, itmax);
+    hipLaunchKernelGGL(poisson_init_kernel, dim3(nb), dim3(kBlock), 0, st, psi0, ctl, diag, count, tol, itmax,
+                       zero);
+    return launched();
+}
+
+int warm_start_launch(double* psi0, const double* psi1, const SorCtl* fin, int n, int ld, hipStream_t st) {
+    const long long count = (long long)n * ld;
+    unsigned nb = blocks_for(count);
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(warm_start_kernel, dim3(nb), dim3(kBlock), 0, st, psi0, psi1, fin, count);
     return launched();
 }
 

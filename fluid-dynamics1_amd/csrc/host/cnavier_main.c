@@ -52,7 +52,8 @@ static void help_config(void)
     printf("Added keys: sor_omega (SOR relaxation; default 0.5*(2/(1+sin(PI/(nx+1)))+2/(1+sin(PI/(ny+1))))),\n");
     printf("            nsteps (default (int)(tf/dt-1)+1), precision (fp64 | fp32dc), inner_rtol (1e-5),\n");
     printf("            inner_beta (fp32dc inner relaxation: 0 = sor_omega, or \"reference\"),\n");
-    printf("            sweeps_per_launch (0 = tuned default), diagnostics (1), device (0), use_graph (0)\n");
+    printf("            sweeps_per_launch (0 = tuned default), diagnostics (1), device (0), use_graph (0),\n");
+    printf("            warm_start (0; 1 = each Poisson solve starts from the previous psi, not 0)\n");
 }
 
 static mtrx to_mtrx(const double *a, int n)
@@ -135,6 +136,7 @@ int main(int argc, char *argv[])
     p.sweeps_per_launch = cx.sweeps_per_launch;
     p.diagnostics = cx.diagnostics;
     p.use_graph = cx.use_graph;
+    p.warm_start = cx.warm_start;
     cnv_ctx *ctx = NULL;
     int rc = cnv_create(&p, &ctx);
     if (rc != CNV_OK) {

@@ -64,6 +64,10 @@ typedef struct {
                               launches, chunks of launches_per_poll SOR launches, the post-solve
                               launches); results identical to the eager launches              */
     double inner_beta;     /* FP32_DC: relaxation of the inner FP32 SOR solves (0 = beta)     */
+    int warm_start;        /* 1 = each Poisson solve starts from the previous step's psi
+                              instead of 0 (src/poisson.c:229 starts from 0): fewer sweeps,
+                              results that differ within the tolerance; off for parity.
+                              One-cavity contexts only (a batch refuses it)                  */
 } cnv_params;
 
 typedef struct {

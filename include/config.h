@@ -43,6 +43,7 @@ typedef struct {
     int use_graph;          /* use_graph: 1 = replay each FP64 step from captured HIP graphs    */
     double inner_beta;      /* inner_beta: relaxation of the inner FP32 solves of fp32dc (0 =
                                sor_omega; "reference" = the reference's formula for nx, ny)     */
+    int warm_start;         /* warm_start: 1 = each Poisson solve starts from the previous psi  */
     int has_sor_omega;
 } ConfigExt;
 
