@@ -55,6 +55,29 @@ def test_fullsize_step_properties():
         assert err <= 1e-6, (f, err)
 
 
+def test_fullsize_step_bit_exact_reference_beta():
+    # the metric's cavity (1024^2, Re 1000, dt = 50*2^-20, order 6, tol 1e-10) with the reference's
+    # own relaxation rule (src/main.c:134, beta = 1.9939 here): about 5.5k sweeps per step, which
+    # the oracle's OpenMP red-black sweeps finish in seconds.  Two steps, bit for bit: 240 tiles
+    # per launch, the convergence decision over their partials, the rollback into the first sweep
+    # with e < tol, all at full size.
+    import os
+    beta = cnv.reference_beta(N)
+    kw = dict(Re=RE, dt=DT, order=6, tol=TOL, itmax=2_000_000)
+    threads = min(16, os.cpu_count() or 1)
+    print("1024^2 reference-beta steps, oracle on %d threads" % threads, flush=True)
+    s_or = orc.State(orc.make_params(N, ordering=orc.REDBLACK, beta_=beta, threads=threads, **kw))
+    per = s_or.run(2)
+    with cnv.Cavity(cnv.default_params(N, beta=beta, **kw)) as cav:
+        st = cav.step(2)
+        s = cav.get_state()
+    print("k", [x.k for x in st], [p.k for p in per], flush=True)
+    assert [x.k for x in st] == [p.k for p in per]
+    assert all(abs(x.e - p.e) <= 1e-12 * p.e for x, p in zip(st, per))
+    for name in ("w", "psi", "u", "v"):
+        assert np.array_equal(s[name], getattr(s_or, name)), name
+
+
 @pytest.mark.parametrize("n", [4096, 4099])
 def test_large_grid_poisson_iterates_bit_exact(n):
     # 7 sweeps on a random source: 4096 takes the paired-access, exact-reciprocal path, 4099 the
