@@ -1,6 +1,7 @@
-"""GPU tests at the workload's full size and beyond, where the oracle cannot run whole steps:
-size-independent properties of a 1024^2 step (the metric's cavity), and bit-exact Poisson
-iterates on 4096^2-class grids (thousands of workgroups, several per CU, both access paths)."""
+"""GPU tests at the workload's full size and beyond: 1024^2 steps bit-exact against the oracle
+(the metric's first step, and two steps with the reference's relaxation rule), size-independent
+properties of a 1024^2 step, and bit-exact Poisson iterates on 4096^2-class grids (thousands of
+workgroups, several per CU, both access paths)."""
 import numpy as np
 import pytest
 
@@ -74,6 +75,25 @@ def test_fullsize_step_bit_exact_reference_beta():
     print("k", [x.k for x in st], [p.k for p in per], flush=True)
     assert [x.k for x in st] == [p.k for p in per]
     assert all(abs(x.e - p.e) <= 1e-12 * p.e for x, p in zip(st, per))
+    for name in ("w", "psi", "u", "v"):
+        assert np.array_equal(s[name], getattr(s_or, name)), name
+
+
+def test_fullsize_metric_step_bit_exact():
+    # the metric's configuration exactly (omega = 1.95): the first step from rest, 48221 sweeps
+    # (about 9.6k launches of 5), against the oracle's OpenMP red-black step (about 35 s on 16
+    # host threads), bit for bit
+    import os
+    kw = dict(Re=RE, dt=DT, order=6, tol=TOL, itmax=50_000_000)
+    threads = min(16, os.cpu_count() or 1)
+    print("1024^2 omega 1.95 step, oracle on %d threads" % threads, flush=True)
+    s_or = orc.State(orc.make_params(N, ordering=orc.REDBLACK, beta_=OMEGA, threads=threads, **kw))
+    per = s_or.run(1)
+    with cnv.Cavity(cnv.default_params(N, beta=OMEGA, use_graph=1, **kw)) as cav:
+        st = cav.step(1)
+        s = cav.get_state()
+    print("sweeps", st[0].sweeps, per[0].sweeps, flush=True)
+    assert st[0].k == per[0].k and abs(st[0].e - per[0].e) <= 1e-12 * per[0].e
     for name in ("w", "psi", "u", "v"):
         assert np.array_equal(s[name], getattr(s_or, name)), name
 
