@@ -136,7 +136,10 @@ int dc_residual_launch(const double* w, const double* psi, float* r32, Diag* dia
                        hipStream_t st);
 int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCtl* fin, int n,
                       int ld, hipStream_t st);
+// budget != nullptr: the inner solve's sweep budget is read from that device counter
 int dc_inner_init_launch(float* e0, SorCtl* ctl, int n, int ld, double rtol, double tol,
-                         int itmax, hipStream_t st);
+                         int itmax, hipStream_t st, const int* budget = nullptr);
+// the device budget counter: fin == nullptr sets it to itmax, else subtracts fin->k
+int dc_budget_launch(int* budget, int itmax, const SorCtl* fin, hipStream_t st);
 
 }  // namespace cnv

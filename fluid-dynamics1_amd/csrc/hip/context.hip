@@ -223,7 +223,8 @@ static int initial_state(cnv_ctx* c) {
 }
 
 static void drop_graphs(cnv_ctx* c) {
-    hipGraphExec_t* gs[] = {&c->g_pre[0], &c->g_pre[1], &c->g_chunk[0], &c->g_chunk[1], &c->g_post};
+    hipGraphExec_t* gs[] = {&c->g_pre[0],  &c->g_pre[1],   &c->g_chunk[0], &c->g_chunk[1], &c->g_tail[0],
+                            &c->g_tail[1], &c->g_resid[0], &c->g_resid[1], &c->g_correct,  &c->g_post};
     for (hipGraphExec_t* g : gs) {
         if (*g) (void)hipGraphExecDestroy(*g);
         *g = nullptr;
@@ -235,7 +236,7 @@ void cnv_destroy(cnv_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
     void* dev[] = {c->w[0], c->w[1], c->psi[0], c->psi[1], c->u, c->v, c->d_start, c->d_tab,
-                   c->d_ftab, c->ctl, c->diag, c->partial, c->e32[0], c->e32[1], c->r32};
+                   c->d_ftab, c->ctl, c->diag, c->partial, c->e32[0], c->e32[1], c->r32, c->d_budget};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
@@ -314,6 +315,7 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
         if (p->precision == CNV_PREC_FP32_DC) {
             for (int b = 0; b < 2; ++b) HIPC(c, hipMalloc(&c->e32[b], sizeof(float) * c->cells));
             HIPC(c, hipMalloc(&c->r32, sizeof(float) * c->cells));
+            HIPC(c, hipMalloc(&c->d_budget, sizeof(int)));
         }
         int rc = upload_tables(c);
         if (rc) return rc;
@@ -378,10 +380,12 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) 
     return CNV_OK;
 }
 
-// ---- captured step (use_graph, FP64) -------------------------------------------------------
+// ---- captured step (use_graph) -------------------------------------------------------------
 // Launch index L only matters through its parity and every solve's state lives in the device
-// slots, so one captured chunk of M (even) SOR launches with L = 0..M-1 is replayed until the
-// solve ends: each replay continues the previous one's slot parity, and the final slot is 1.
+// slots, so a captured chunk of M (even) SOR launches with L = 0..M-1 is replayed until the
+// solve ends: each replay continues the previous one's slot parity, and after whole chunks the
+// final slot is 1.  Two chunk sizes: launches_per_poll for the predicted length of the solve
+// and kTailChunk for its end, so that the no-op launches queued after convergence stay few.
 static int capture(cnv_ctx* c, hipGraphExec_t* out, int (*body)(cnv_ctx*, int), int arg) {
     hipGraph_t g = nullptr;
     HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -398,10 +402,21 @@ static int capture(cnv_ctx* c, hipGraphExec_t* out, int (*body)(cnv_ctx*, int), 
     return CNV_OK;
 }
 
+static bool is_dc(const cnv_ctx* c) { return c->p.precision == CNV_PREC_FP32_DC; }
+
 // wall vorticity and transport of w[wc] into w[wc^1], then a fresh Poisson solve
 static int body_pre(cnv_ctx* c, int wc) {
     const int n = c->n, ld = c->ld;
     LAUNCH(c, wall_vorticity_launch(c->w[wc], c->u, c->v, n, ld, c->st, c->bc, c->stream));
+    if (is_dc(c)) {
+        LAUNCH(c, transport_fp32_launch(c->w[wc], c->w[wc ^ 1], c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re,
+                                        c->p.dt, c->stream));
+        // psi[0] is the FP64 accumulator (a warm start keeps it); the solve's sweep budget
+        LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, n, ld, c->p.tol, c->p.itmax, c->stream,
+                                      !c->p.warm_start));
+        LAUNCH(c, dc_budget_launch(c->d_budget, c->p.itmax, nullptr, c->stream));
+        return CNV_OK;
+    }
     LAUNCH(c, transport_fp64_launch(c->w[wc], c->w[wc ^ 1], c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re, c->p.dt,
                                     c->stream));
     // warm start: whole chunks end on slot 1, which holds the previous solve's final state
@@ -410,47 +425,93 @@ static int body_pre(cnv_ctx* c, int wc) {
                                   !c->p.warm_start));
     return CNV_OK;
 }
-// M SOR launches, L = 0..M-1, on the source w[src] (f = -w)
-static int body_chunk(cnv_ctx* c, int src) {
-    const SorArgs<double> a = sor_args<double>(c, c->w[src], c->psi[0], c->psi[1], 1);
+// m SOR launches, L = 0..m-1: FP64 on the source w[src] (f = -w), or the refined form's FP32
+// inner solve on r32
+static int body_chunk_of(cnv_ctx* c, int src, int m) {
     const bool gs = c->p.poisson_type == 1;
-    for (int l = 0; l < c->launches_per_poll; ++l)
+    if (is_dc(c)) {
+        const SorArgs<float> a = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
+        for (int l = 0; l < m; ++l)
+            if (launch_sor(c, a, l, gs)) return cnv_fail(c, CNV_ERR_HIP, "SOR launch failed while capturing");
+        return CNV_OK;
+    }
+    const SorArgs<double> a = sor_args<double>(c, c->w[src], c->psi[0], c->psi[1], 1);
+    for (int l = 0; l < m; ++l)
         if (launch_sor(c, a, l, gs)) return cnv_fail(c, CNV_ERR_HIP, "SOR launch failed while capturing");
     return CNV_OK;
 }
-// velocities from the final iterate (slot 1 after whole chunks), continuity diagnostic
+static int body_chunk(cnv_ctx* c, int src) { return body_chunk_of(c, src, c->launches_per_poll); }
+static int body_tail(cnv_ctx* c, int src) { return body_chunk_of(c, src, cnv_ctx::kTailChunk); }
+// refined form, one outer iteration: FP64 residual of psi[0] against w[src], inner solve init
+// with the remaining budget (device counter) ...
+static int body_resid(cnv_ctx* c, int src) {
+    const double dx = c->p.dx, dy = c->p.dy;
+    LAUNCH(c, dc_residual_launch(c->w[src], c->psi[0], c->r32, c->diag, c->n, c->ld, dy * dy, dx * dx,
+                                 dx * dx * dy * dy, 2 * (dx * dx + dy * dy), c->p.beta, c->stream));
+    LAUNCH(c, dc_inner_init_launch(c->e32[0], c->ctl, c->n, c->ld, c->p.inner_rtol, c->p.tol, 0, c->stream,
+                                   c->d_budget));
+    return CNV_OK;
+}
+// ... and after the inner solve (final slot 1): psi[0] += e, budget -= its sweeps
+static int body_correct(cnv_ctx* c, int) {
+    LAUNCH(c, dc_correct_launch(c->psi[0], c->e32[0], c->e32[1], c->ctl + 1, c->n, c->ld, c->stream));
+    LAUNCH(c, dc_budget_launch(c->d_budget, 0, c->ctl + 1, c->stream));
+    return CNV_OK;
+}
+// velocities from the final iterate (slot 1 after whole chunks; psi[0] in the refined form),
+// continuity diagnostic
 static int body_post(cnv_ctx* c, int) {
-    LAUNCH(c, velocity_launch(c->psi[0], c->psi[1], c->ctl + 1, c->u, c->v, c->n, c->ld, c->st, c->stream));
+    const double* p1 = is_dc(c) ? c->psi[0] : c->psi[1];
+    LAUNCH(c, velocity_launch(c->psi[0], p1, c->ctl + 1, c->u, c->v, c->n, c->ld, c->st, c->stream));
     if (c->p.diagnostics) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, c->n, c->ld, c->st, c->stream));
     return CNV_OK;
 }
 
 static int build_graphs(cnv_ctx* c) {
     if (c->launches_per_poll & 1) ++c->launches_per_poll;  // whole chunks end on slot 1
+    const int nsrc = is_dc(c) ? 1 : 2;  // the refined form's inner solves always read r32
     for (int wc = 0; wc < 2; ++wc) {
         int rc = capture(c, &c->g_pre[wc], body_pre, wc);
-        if (!rc) rc = capture(c, &c->g_chunk[wc], body_chunk, wc);
+        if (!rc && wc < nsrc) rc = capture(c, &c->g_chunk[wc], body_chunk, wc);
+        if (!rc && wc < nsrc) rc = capture(c, &c->g_tail[wc], body_tail, wc);
+        if (!rc && is_dc(c)) rc = capture(c, &c->g_resid[wc], body_resid, wc);
+        if (rc) return rc;
+    }
+    if (is_dc(c)) {
+        const int rc = capture(c, &c->g_correct, body_correct, 0);
         if (rc) return rc;
     }
     return capture(c, &c->g_post, body_post, 0);
 }
 
-// replays the chunk graph of source plane src until a polled slot is terminal
-static int run_sor_graph(cnv_ctx* c, int src, SorCtl* final_ctl) {
-    const int M = c->launches_per_poll;
+// Replays chunk graphs until a polled slot is terminal.  With a prediction of P launches the
+// first big chunks cover about P - 2*kTailChunk launches and tail chunks follow, as many per
+// poll as make up the eager solve's chunk for that prediction (poll_chunk); without one, big
+// chunks one per poll.
+static int replay_solve(cnv_ctx* c, int src, int predicted_sweeps, SorCtl* final_ctl) {
+    const int M = c->launches_per_poll, Mt = cnv_ctx::kTailChunk;
     const int S = c->shape.s;
-    const long long cap = (long long)c->p.itmax + 4LL * M + 8;
-    int first = 1;  // chunks queued before the first poll
-    if (c->k_prev > 0) first = std::max(1, (c->k_prev / S - M + M - 1) / M);
+    const int P = predicted_sweeps > 0 ? predicted_sweeps / S : 0;
+    const long long cap = (long long)c->p.itmax / S + 4LL * M + 8;
+    const int first = P > 0 ? std::max(0, (P - 2 * Mt) / M) : 1;  // big chunks before the first poll
+    const int tails = P > 0 ? std::max(1, (poll_chunk(c, P) + Mt / 2) / Mt) : 0;  // tail chunks per poll
     long long queued = 0;
-    int slot = 0;
+    int slot = 0, round = 0;
     int pending[kSlots];
     int npend = 0;
     c->fin_slot = 1;
     while (true) {
-        const int chunks = queued == 0 ? first : 1;
-        for (int k = 0; k < chunks; ++k) HIPC(c, hipGraphLaunch(c->g_chunk[src], c->stream));
-        queued += (long long)chunks * M;
+        if (round == 0 && first > 0) {
+            for (int k = 0; k < first; ++k) HIPC(c, hipGraphLaunch(c->g_chunk[src], c->stream));
+            queued += (long long)first * M;
+        } else if (P > 0) {
+            for (int k = 0; k < tails; ++k) HIPC(c, hipGraphLaunch(c->g_tail[src], c->stream));
+            queued += (long long)tails * Mt;
+        } else {
+            HIPC(c, hipGraphLaunch(c->g_chunk[src], c->stream));
+            queued += M;
+        }
+        ++round;
         HIPC(c, hipMemcpyAsync(&c->h_ctl[slot], c->ctl + 1, sizeof(SorCtl), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipEventRecord(c->ev[slot], c->stream));
         pending[npend++] = slot;
@@ -471,6 +532,30 @@ static int run_sor_graph(cnv_ctx* c, int src, SorCtl* final_ctl) {
     }
 }
 
+// the refined form's solve from captured graphs: outer iterations driven by the host, each a
+// residual graph, replayed inner chunks and a correction graph (as poisson_dc)
+static int poisson_dc_graph(cnv_ctx* c, int src, SorCtl* fin, int* outer) {
+    int total = 0;
+    const int max_outer = c->p.max_outer > 0 ? c->p.max_outer : 50;
+    *fin = SorCtl{};
+    fin->status = SOR_FAILED;
+    for (int o = 0; o < max_outer && total < c->p.itmax; ++o) {
+        HIPC(c, hipGraphLaunch(c->g_resid[src], c->stream));
+        SorCtl snap{};
+        int rc = replay_solve(c, 0, o < cnv_ctx::kDcPred ? c->k_dc[o] : 0, &snap);
+        if (rc) return rc;
+        if (o < cnv_ctx::kDcPred) c->k_dc[o] = snap.k;
+        HIPC(c, hipGraphLaunch(c->g_correct, c->stream));
+        total += snap.k;
+        *outer = o + 1;
+        *fin = snap;
+        fin->k = total;
+        if (snap.status == SOR_CONVERGED || snap.status == SOR_FAILED) return CNV_OK;
+    }
+    fin->status = SOR_FAILED;
+    return CNV_OK;
+}
+
 static int one_step(cnv_ctx* c, cnv_step_stats* stats);
 
 static int one_step_graph(cnv_ctx* c, cnv_step_stats* stats) {
@@ -484,15 +569,17 @@ static int one_step_graph(cnv_ctx* c, cnv_step_stats* stats) {
     HIPC(c, hipGraphLaunch(c->g_pre[c->wcur], c->stream));
     c->wcur ^= 1;
     SorCtl fin{};
-    int rc = run_sor_graph(c, c->wcur, &fin);
+    int outer = 0;
+    int rc = is_dc(c) ? poisson_dc_graph(c, c->wcur, &fin, &outer) : replay_solve(c, c->wcur, c->k_prev, &fin);
     if (rc) return rc;
-    c->k_prev = fin.k;
+    if (!is_dc(c)) c->k_prev = fin.k;
     HIPC(c, hipGraphLaunch(c->g_post, c->stream));
     cnv_step_stats s{};
     s.status = fin.status == SOR_CONVERGED ? CNV_OK : CNV_NOT_CONVERGED;
     s.sweeps = fin.k;
     s.k = fin.k - 1;
     s.e = fin.e_last;
+    s.outer = outer;
     if (c->p.diagnostics) {
         HIPC(c, hipMemcpyAsync(c->h_diag, c->diag, sizeof(Diag), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
@@ -506,7 +593,7 @@ static int one_step_graph(cnv_ctx* c, cnv_step_stats* stats) {
 }
 
 static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
-    if (c->p.use_graph && c->p.precision == CNV_PREC_FP64) return one_step_graph(c, stats);
+    if (c->p.use_graph) return one_step_graph(c, stats);
     const int n = c->n, ld = c->ld;
     const double rRe = 1. / c->p.Re;  // src/fluiddyn.c:98 "(1. / Re)"
     double* w_in = c->w[c->wcur];

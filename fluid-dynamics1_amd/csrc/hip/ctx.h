@@ -49,11 +49,18 @@ struct cnv_ctx {
     static constexpr int kDcPred = 64;
     int k_dc[kDcPred] = {};  // FP32_DC: sweeps of the inner solve of each outer iteration
     cnv::WallBC bc{};
-    // captured step (use_graph, FP64): pre-solve launches per w parity, a chunk of
-    // launches_per_poll SOR launches per source plane, post-solve launches
+    // captured step (use_graph): pre-solve launches per w parity, a chunk of launches_per_poll
+    // SOR launches and a tail chunk of kTailChunk per source plane (FP64; the refined form's
+    // inner solves use index 0), post-solve launches; the refined form adds, per outer
+    // iteration, a residual + inner-init graph per w parity and a correction graph
+    static constexpr int kTailChunk = 4;
     hipGraphExec_t g_pre[2] = {nullptr, nullptr};
     hipGraphExec_t g_chunk[2] = {nullptr, nullptr};
+    hipGraphExec_t g_tail[2] = {nullptr, nullptr};
+    hipGraphExec_t g_resid[2] = {nullptr, nullptr};
+    hipGraphExec_t g_correct = nullptr;
     hipGraphExec_t g_post = nullptr;
+    int* d_budget = nullptr;  // refined form, captured: the solve's remaining sweep budget
     char err[256] = "";
 };
 

@@ -50,10 +50,18 @@ __global__ void __launch_bounds__(kBlock) dc_correct_kernel(double* __restrict__
 // start of an inner solve: e = 0, control block reset with the absolute tolerance (the plain
 // solver's stopping rule) and the FP32 fold ratio
 __global__ void __launch_bounds__(kBlock) dc_inner_init_kernel(float* e0, SorCtl* ctl, long long count,
-                                                              double rtol, double tol, int itmax) {
+                                                              double rtol, double tol, int itmax,
+                                                              const int* budget) {
     const long long stride = (long long)gridDim.x * kBlock;
     for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) e0[q] = 0.0f;
-    if (blockIdx.x == 0 && threadIdx.x < 2) ctl_reset(ctl + threadIdx.x, tol, rtol, itmax);
+    // the sweep budget comes from the device counter when there is one (captured step)
+    if (blockIdx.x == 0 && threadIdx.x < 2) ctl_reset(ctl + threadIdx.x, tol, rtol, budget ? *budget : itmax);
+}
+
+// device sweep budget of a refined solve: fin == nullptr resets it to itmax, otherwise the inner
+// solve that ended in slot fin spent fin->k sweeps of it
+__global__ void dc_budget_kernel(int* budget, int itmax, const SorCtl* fin) {
+    if (threadIdx.x == 0) *budget = fin ? *budget - fin->k : itmax;
 }
 
 __global__ void __launch_bounds__(kBlock) transport_fp32_kernel(const double* __restrict__ w,
@@ -85,11 +93,17 @@ int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCt
 }
 
 int dc_inner_init_launch(float* e0, SorCtl* ctl, int n, int ld, double rtol, double tol, int itmax,
-                         hipStream_t st) {
+                         hipStream_t st, const int* budget) {
     const long long count = (long long)n * ld;
     unsigned nb = blocks_for(count);
     if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(dc_inner_init_kernel, dim3(nb), dim3(kBlock), 0, st, e0, ctl, count, rtol, tol, itmax);
+    hipLaunchKernelGGL(dc_inner_init_kernel, dim3(nb), dim3(kBlock), 0, st, e0, ctl, count, rtol, tol, itmax,
+                       budget);
+    return launched();
+}
+
+int dc_budget_launch(int* budget, int itmax, const SorCtl* fin, hipStream_t st) {
+    hipLaunchKernelGGL(dc_budget_kernel, dim3(1), dim3(64), 0, st, budget, itmax, fin);
     return launched();
 }
 

@@ -60,9 +60,11 @@ typedef struct {
                               (0: per solve from the previous solve's length, 64 per captured
                               chunk with use_graph, 2 for a batch)                          */
     int diagnostics;       /* 1 = continuity max/min every step (src/main.c:387-408)        */
-    int use_graph;         /* FP64: 1 = each step replays captured HIP graphs (the pre-solve
-                              launches, chunks of launches_per_poll SOR launches, the post-solve
-                              launches); results identical to the eager launches              */
+    int use_graph;         /* 1 = each step replays captured HIP graphs (the pre-solve
+                              launches, chunks of launches_per_poll SOR launches and tail
+                              chunks of 4, the post-solve launches; FP32_DC also the residual
+                              and correction of each outer iteration, with the sweep budget
+                              kept on the device); results identical to the eager launches   */
     double inner_beta;     /* FP32_DC: relaxation of the inner FP32 SOR solves (0 = beta)     */
     int warm_start;        /* 1 = each Poisson solve starts from the previous step's psi
                               instead of 0 (src/poisson.c:229 starts from 0): fewer sweeps,

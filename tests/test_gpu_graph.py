@@ -1,6 +1,7 @@
 """GPU tests of the captured step (use_graph = 1): every step replays HIP graphs (pre-solve
-launches, chunks of SOR launches, post-solve launches) and must give exactly what the eager
-launches give, step after step, with eager work and host synchronisation between the replays."""
+launches, chunks of SOR launches, post-solve launches; in the refined form also the residual
+and correction of each outer iteration) and must give exactly what the eager launches give,
+step after step, with eager work and host synchronisation between the replays."""
 import numpy as np
 import pytest
 
@@ -34,3 +35,32 @@ def test_graph_multi_step_call():
         assert [s.k for s in se] == [s.k for s in sg]
         a, b = eager.get_state(), graph.get_state()
         assert all(np.array_equal(a[f], b[f]) for f in ("w", "psi", "u", "v"))
+
+
+@pytest.mark.parametrize("warm", [0, 1], ids=["cold", "warm"])
+def test_graph_refined_step_equals_eager(warm):
+    # the refined form: per outer iteration a residual graph, replayed FP32 inner chunks (big and
+    # tail) and a correction graph, the sweep budget on the device
+    kw = dict(Re=1000.0, dt=1e-3, tol=1e-10, itmax=2_000_000, beta=1.9, precision=cnv.PREC_FP32_DC,
+              warm_start=warm, ui=0.1, vi=-0.05, u=(0.2, -0.1, 0.05, 1.2), v=(-0.3, 0.1, 0.25, -0.15))
+    print("refined graph case warm=%d" % warm, flush=True)
+    with cnv.Cavity(cnv.default_params(96, **kw)) as eager, \
+            cnv.Cavity(cnv.default_params(96, use_graph=1, **kw)) as graph:
+        for t in range(4):
+            se = eager.step(1)[0]
+            sg = graph.step(1)[0]
+            assert (sg.status, sg.sweeps, sg.outer, sg.e) == (se.status, se.sweeps, se.outer, se.e), t
+            a, b = eager.get_state(), graph.get_state()
+            for f in ("w", "psi", "u", "v"):
+                assert np.array_equal(a[f], b[f]), (t, f)
+
+
+def test_graph_refined_budget():
+    # a budget too small to converge: the device counter stops the inner solves, as the host does
+    kw = dict(Re=1000.0, dt=2e-3, tol=1e-12, itmax=50, precision=cnv.PREC_FP32_DC)
+    with cnv.Cavity(cnv.default_params(64, **kw)) as eager, \
+            cnv.Cavity(cnv.default_params(64, use_graph=1, **kw)) as graph:
+        se = eager.step(1, check=False)[0]
+        sg = graph.step(1, check=False)[0]
+    assert se.status == sg.status == cnv.NOT_CONVERGED
+    assert sg.sweeps == se.sweeps
