@@ -19,7 +19,8 @@
 //     keep psi = 0: their relaxation factor is 0 instead of beta, which leaves them at +-0, and
 //     the final store writes +0 there;
 //   * the L1 norms are accumulated per lane with a 0/1 weight per row (owned rows only) and a
-//     per-column ownership mask, written to LDS once per sweep and reduced once per launch.
+//     per-column ownership mask, written to LDS once per sweep and reduced once per launch
+//     (FP32 planes: in the launch's last sweep only).
 // Point update (src/poisson.c:268 SOR, src/poisson.c:205 Gauss-Seidel), same association:
 //   u = beta*(dy*dy*(N+S) + dx*dx*(E+W) - dx*dx*dy*dy*f) / (2*(dx*dx+dy*dy)) + (1-beta)*u0
 // where u0 (the value before this sweep) is the point's current value in either colouring.
@@ -38,6 +39,7 @@
 //   * if sweep s < S-1 of the previous launch reached e < tol, that launch's output is
 //     discarded too: this launch redoes s+1 sweeps from its (untouched) input buffer, so the
 //     final iterate is exactly the one of the first sweep with e < tol (src/poisson.c:273);
+//     FP32 planes, a preconditioner, test the last sweep of each launch only;
 //   * the sweep budget itmax is honoured exactly (src/poisson.c:234);
 //   * relative rule: with rtol > 0 the solve also stops ("fold") once the update norm of a sweep
 //     fell below rtol times the iterate's own L1 norm after that launch's last sweep (measured
@@ -179,6 +181,12 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     // the iterate's L1 norm is needed by the relative stopping rule (rtol): always on FP32
     // planes, on request (NORM) for FP64.  It is measured once per launch, after its last sweep.
     constexpr bool kNorm = sizeof(T) == 4 || NORM;
+    // FP32 planes (the inner solves of defect correction) are a preconditioner with no bit-parity
+    // requirement: their stopping rule is applied to each launch's last sweep only, so the other
+    // sweeps skip the L1 update (a solve runs at most S-1 sweeps past the first sweep that met
+    // the rule).  FP64 applies it to every sweep, as src/poisson.c:272-278 does.
+    constexpr bool kLastOnly = sizeof(T) == 4;
+    constexpr int SE = kLastOnly ? 1 : S;  // L1 update sums per launch
     constexpr int EJ = 128;          // extended tile columns; lane m holds columns 2m, 2m+1
     constexpr int TJ = EJ - 2 * H;
     constexpr int EI = NW * RB;      // extended tile rows; wave w holds rows [w*RB, w*RB+RB)
@@ -188,11 +196,11 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     typedef typename std::conditional<sizeof(T) == 4, float, double>::type Acc;
 
     __shared__ T xch[2][NW][2][64];  // [half-sweep parity][wave][first/last row][lane]
-    // one column per thread: this launch's per-lane L1 update sums, one row per sweep, and the
-    // per-lane iterate norm after the last sweep
-    __shared__ double lred[S][NW * 64];
+    // one column per thread: this launch's per-lane L1 update sums, one row per summed sweep,
+    // and the per-lane iterate norm after the last sweep
+    __shared__ double lred[SE][NW * 64];
     __shared__ double lnrm[kNorm ? NW * 64 : 1];
-    constexpr int nq = S + (kNorm ? 1 : 0);  // reduced quantities: S updates (and the norm)
+    constexpr int nq = SE + (kNorm ? 1 : 0);  // reduced quantities: SE updates (and the norm)
     __shared__ double lw[NW][nq];       // per-wave totals of the previous launch's partials
 
     const int L = a.launch;
@@ -231,8 +239,10 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     else bcoef = GS ? a.rB : a.beta * a.rB;
 
     T v[2][RB];   // v[q][r]: value at (row0 + r, 2*lane + q)
-    T fc[2][RB];  // FP64: c3*f (the hoisted c3*f product of the update); FP32: -c3*f
-    T be[2][RB];  // bcoef at interior points, 0 on the domain boundary and outside the domain
+    T fc[2][RB];  // FP64: c3*f (the hoisted c3*f product of the update); FP32: -be*c3*f
+    T be[2][RB];  // FP64: bcoef at interior points, 0 on the domain boundary and outside it
+    T cn[2][RB];  // FP32: be*c1 and be*c2, the relaxation folded into the stencil coefficients
+    T ce[2][RB];
     Acc w[RB];    // 1 for owned rows inside the domain, else 0 (wave-uniform)
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
@@ -250,16 +260,23 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
             const bool inter = rint && colin[q];
             const T fq = q ? f1 : f0;
             const T fv = a.negate ? -fq : fq;
-            if constexpr (sizeof(T) == 8) fc[q][r] = a.c3 * fv;
-            else fc[q][r] = -(a.c3 * fv);
-            be[q][r] = inter ? bcoef : T(0);
+            const T bq = inter ? bcoef : T(0);
+            if constexpr (sizeof(T) == 8) {
+                fc[q][r] = a.c3 * fv;
+                be[q][r] = bq;
+            } else {
+                fc[q][r] = bq * -(a.c3 * fv);
+                cn[q][r] = bq * a.c1;
+                ce[q][r] = bq * a.c2;
+            }
         }
     }
 
     // the previous launch's per-tile partials: thread t sums tiles t, t + NT, ... in that order
     // (all S slots are read, before the state slot is known, so that these loads are in flight
     // together with the tile and the slot; only the first sd are used).  Slot layout per tile:
-    // [s][tile][0] the L1 update of sweep s, [0][tile][1] the iterate's norm after the last sweep.
+    // [s][tile][0] the L1 update of sweep s (FP32: [0][tile][0], the launch's last sweep),
+    // [0][tile][1] the iterate's norm after the last sweep.
     const size_t pstride = (size_t)kMaxSweepsPerLaunch * ntiles * 2;
     double pv[nq];  // pv[s]: updates; pv[S]: norm
 #pragma unroll
@@ -270,7 +287,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         if (ntiles <= NT) {  // one tile per thread at most (1024^2: 240 tiles)
             if (tid < ntiles) {
 #pragma unroll
-                for (int s = 0; s < S; ++s) pv[s] = pp[((size_t)s * ntiles + tid) * 2];
+                for (int s = 0; s < SE; ++s) pv[s] = pp[((size_t)s * ntiles + tid) * 2];
                 if (kNorm) pv[nq - 1] = pp[(size_t)tid * 2 + 1];
             }
         } else {
@@ -287,7 +304,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
                     const int t = t0 + u * NT;
                     const int tc = t < ntiles ? t : 0;  // in range; its value is discarded
 #pragma unroll
-                    for (int s = 0; s < S; ++s) x[u][s] = pp[((size_t)s * ntiles + tc) * 2];
+                    for (int s = 0; s < SE; ++s) x[u][s] = pp[((size_t)s * ntiles + tc) * 2];
                     if (kNorm) x[u][nq - 1] = pp[(size_t)tc * 2 + 1];
                 }
 #pragma unroll
@@ -327,6 +344,7 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         __syncthreads();
     };
     // sweep s (red half-sweep, then black) of a pass; with_norm: also sum the iterate's norm
+    // (FP32: and the update, which only the last sweep sums)
     auto one_sweep = [&](const int s, const bool with_norm) {
         Acc acc[2] = {Acc(0), Acc(0)}, nrm[2] = {Acc(0), Acc(0)};
 #pragma unroll
@@ -357,12 +375,14 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
                     acc[q] = __fma_rn(w[r], fabs(cand - old), acc[q]);
                     if (kNorm && with_norm) nrm[q] = __fma_rn(w[r], fabs(cand), nrm[q]);
                 } else {
-                    const T x = __fmaf_rn(c1, up + dn, __fmaf_rn(c2, own + far, fc[q][r]));
-                    const T y = be[q][r] * x;
-                    cand = GS ? y : __fmaf_rn(omb, old, y);
+                    // relaxation and 1/B folded into cn, ce and fc (0 where the point is fixed)
+                    const T x = __fmaf_rn(cn[q][r], up + dn, __fmaf_rn(ce[q][r], own + far, fc[q][r]));
+                    cand = GS ? x : __fmaf_rn(omb, old, x);
                     v[q][r] = cand;
-                    acc[q] = __fmaf_rn(w[r], fabsf(cand - old), acc[q]);
-                    if (with_norm) nrm[q] = __fmaf_rn(w[r], fabsf(cand), nrm[q]);
+                    if (with_norm) {
+                        acc[q] = __fmaf_rn(w[r], fabsf(cand - old), acc[q]);
+                        nrm[q] = __fmaf_rn(w[r], fabsf(cand), nrm[q]);
+                    }
                 }
             }
             xch[c][wv][0][lane] = v[c & 1][0];
@@ -370,15 +390,16 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
             __syncthreads();
         }
         // per-lane sums of this sweep, owned columns only; reduced once after the last sweep
-        lred[s][tid] = (double)((ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0)));
+        if (!kLastOnly || with_norm)
+            lred[kLastOnly ? 0 : s][tid] = (double)((ownc[0] ? acc[0] : Acc(0)) + (ownc[1] ? acc[1] : Acc(0)));
         if (kNorm && with_norm)
             lnrm[tid] = (double)((ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0)));
     };
     // a pass of ns sweeps; the rollback pass (rare) and a budget-limited last launch take the
-    // loop, which sums the norm in every sweep (the last one's stays)
+    // loop, which sums the FP64 norm in every sweep (the last one's stays)
     auto sweeps = [&](const int ns) {
         sweeps_begin();
-        for (int s = 0; s < ns; ++s) one_sweep(s, kNorm);
+        for (int s = 0; s < ns; ++s) one_sweep(s, kLastOnly ? s == ns - 1 : kNorm);
     };
     int mode = 0;          // 0: sweep on, 1: stop (terminal), 2: redo s_run sweeps (rollback)
     int s_run = s_spec;
@@ -412,30 +433,31 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     // this launch's per-tile partials (the sweeps' per-lane sums in lred, the norm in lnrm):
     // wave w reduces rows, in a fixed order.  Written whatever the decision below: a launch that
     // ends the solve leaves partials nobody reads.
-    const int nrows = s_spec > 0 ? s_spec + (kNorm ? 1 : 0) : 0;
+    const int nupd = kLastOnly ? 1 : s_spec;  // update rows (FP32: the last sweep's)
+    const int nrows = s_spec > 0 ? nupd + (kNorm ? 1 : 0) : 0;
     for (int row = wv; row < nrows; row += NW) {
-        const double* col = row < s_spec ? lred[row] : lnrm;
+        const double* col = row < nupd ? lred[row] : lnrm;
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < NW; ++j) acc += col[lane + 64 * j];
         acc = wave_sum(acc);
-        const size_t slot = row < s_spec ? ((size_t)row * ntiles + tile) * 2 : (size_t)tile * 2 + 1;
+        const size_t slot = row < nupd ? ((size_t)row * ntiles + tile) * 2 : (size_t)tile * 2 + 1;
         if (lane == 0) a.partial[(L & 1) * pstride + slot] = acc;
     }
 
     // ---- decision on the previous launch (src/poisson.c:272-278): lane i of every wave sums the
     //      per-wave totals of quantity i in wave order, the same in every workgroup, so all reach
     //      the same decision; the totals are then broadcast to the whole wave -------------------
-    double etot[S];      // the previous launch's L1 update of sweep s
+    double etot[SE];     // the previous launch's L1 update of sweep s (FP32: of its last sweep)
     double ntot = 0.0;   // and its iterate's L1 norm after its last sweep (relative rule)
     {
         double e = 0.0;
-        if (sd > 0 && (lane < sd || (kNorm && lane == nq - 1))) {
+        if (sd > 0 && (lane < (kLastOnly ? 1 : sd) || (kNorm && lane == nq - 1))) {
             for (int j = 0; j < nwp; ++j) e += lw[j][lane];
         }
         const int lo = __double2loint(e), hi = __double2hiint(e);
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+        for (int s = 0; s < SE; ++s)
             etot[s] = __hiloint2double(__builtin_amdgcn_readlane(hi, s), __builtin_amdgcn_readlane(lo, s));
         if (kNorm)
             ntot = __hiloint2double(__builtin_amdgcn_readlane(hi, nq - 1), __builtin_amdgcn_readlane(lo, nq - 1));
@@ -452,21 +474,23 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     next.n_last = prev->n_last;
     if (sd > 0) {
         // the first sweep s < sd with e < tol (converged) or, failing that, below rtol times the
-        // iterate's norm after the launch (fold); static indices keep etot in registers
+        // iterate's norm after the launch (fold); static indices keep etot in registers.  Sum i
+        // belongs to sweep i (FP32: to the last sweep, sd - 1, so there is never a rollback).
         int s_conv = -1, s_fold = -1;
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
+        for (int i = 0; i < SE; ++i) {
+            const int s = kLastOnly ? sd - 1 : i;
             if (s < sd && s_conv < 0 && s_fold < 0) {
-                if (etot[s] < tol) s_conv = s;
-                else if (rtol > 0.0 && etot[s] < rtol * ntot) s_fold = s;
+                if (etot[i] < tol) s_conv = s;
+                else if (rtol > 0.0 && etot[i] < rtol * ntot) s_fold = s;
             }
         }
         const int s_rep = (s_conv >= 0 && s_conv < sd - 1) ? s_conv : sd - 1;  // reported sweep
         double e_rep = 0.0;
         const double n_rep = ntot;
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (s == s_rep) e_rep = etot[s];
+        for (int i = 0; i < SE; ++i) {
+            if ((kLastOnly ? sd - 1 : i) == s_rep) e_rep = etot[i];
         }
         if (s_conv >= 0 && s_conv < sd - 1) {
             // converged inside the previous launch: redo s_conv+1 sweeps from its
