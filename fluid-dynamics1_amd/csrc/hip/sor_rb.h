@@ -29,7 +29,8 @@
 // correctly rounded quotient RN(x/B) -- the same double as x/B -- for finite x.  When B is a
 // power of two (dx = dy = 2^-k), 1/B is exact and x*(1/B) is already RN(x/B): one multiply (XR).
 // The L1 norm of each sweep's update (src/poisson.c:34-60; sqrt(pow(d,2)) == |d|) is summed
-// over owned points per workgroup in a fixed order into partial[L&1][s][tile].  The next launch
+// over owned points per workgroup in a fixed order into partial[L&1][s][tile] (FP32 planes: the
+// last sweep's only, into s = 0).  The next launch
 // decides on them after its own (speculative) sweeps, so that neither the partials' latency nor
 // the reduction delays the sweeps: every workgroup reduces the partials in the same fixed order,
 // so all reach the same decision, and tile 0 records it in the state slot L&1 (SorCtl) that the

@@ -49,7 +49,8 @@ typedef struct {
     double u1, u2, u3, u4; /* wall velocities: column 0, column n-1, row 0, row n-1          */
     double v1, v2, v3, v4;
     /* FP32_DC only */
-    double inner_rtol;     /* inner FP32 solve: fold once its L1 update < inner_rtol * L1 norm of the inner iterate */
+    double inner_rtol;     /* inner FP32 solve: fold once its L1 update < inner_rtol * L1 norm of
+                              the inner iterate, tested on the last sweep of each launch      */
     int max_outer;         /* defect-correction iterations allowed per solve                */
     /* tuning (0 = automatic) */
     int sweeps_per_launch; /* SOR sweeps fused per kernel launch (compiled: 1, 2, 4, 5, 6, 8; 0 =
