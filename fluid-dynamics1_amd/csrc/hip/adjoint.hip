@@ -189,7 +189,7 @@ __global__ void adj_zero_kernel(double* p, long long count) {
     if (q < count) p[q] = 0.0;
 }
 
-char g_aerr[256] = "";
+thread_local char g_aerr[256] = "";  // last error of this thread
 
 }  // namespace
 

@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(kBlock) batch_zero_kernel(const StepItem* __re
     }
 }
 
-char g_berr[256] = "";
+thread_local char g_berr[256] = "";  // last error of this thread
 
 }  // namespace
 

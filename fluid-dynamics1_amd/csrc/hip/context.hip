@@ -30,7 +30,7 @@ using namespace cnv;
 
 namespace {
 
-char g_err[256] = "";
+thread_local char g_err[256] = "";  // last error of this thread (contexts may live on different threads)
 
 }  // namespace
 

@@ -91,7 +91,7 @@ double cnv_reference_beta(int nx, int ny);
 
 int cnv_create(const cnv_params *p, cnv_ctx **out);
 void cnv_destroy(cnv_ctx *ctx);
-const char *cnv_last_error(const cnv_ctx *ctx);   /* ctx may be NULL: last global error */
+const char *cnv_last_error(const cnv_ctx *ctx);   /* ctx may be NULL: this thread's last error */
 const char *cnv_status_string(int status);
 
 /* State transfer (host arrays of n*n doubles; any pointer may be NULL to skip a field).
@@ -155,7 +155,7 @@ typedef struct cnv_batch cnv_batch;
 int cnv_batch_create(const cnv_params *items, int count, cnv_batch **out);
 void cnv_batch_destroy(cnv_batch *b);
 int cnv_batch_size(const cnv_batch *b);
-const char *cnv_batch_last_error(const cnv_batch *b);   /* b may be NULL: last batch error */
+const char *cnv_batch_last_error(const cnv_batch *b);   /* b may be NULL: this thread's last batch error */
 /* Advance every cavity by nsteps.  stats: NULL or count*nsteps entries, stats[t*count + i]. */
 int cnv_batch_step(cnv_batch *b, int nsteps, cnv_step_stats *stats);
 int cnv_batch_step_timed(cnv_batch *b, int nsteps, cnv_step_stats *stats, double *elapsed_ms);
@@ -183,7 +183,7 @@ typedef struct cnv_adjoint cnv_adjoint;
 int cnv_adj_create(const cnv_params *p, int checkpoint_every, double rtol_forward, double rtol_adjoint,
                    cnv_adjoint **out);
 void cnv_adj_destroy(cnv_adjoint *a);
-const char *cnv_adj_last_error(const cnv_adjoint *a);   /* a may be NULL */
+const char *cnv_adj_last_error(const cnv_adjoint *a);   /* a may be NULL: this thread's last adjoint error */
 /* Forward run from (w0, U) for nsteps; stores the checkpoints; returns E. */
 int cnv_adj_forward(cnv_adjoint *a, const double *w0, double U, int nsteps, double *energy);
 /* Gradient of the last forward run's E: dE_dw0 (host, n*n) and dE_dU (either may be NULL). */
