@@ -385,6 +385,7 @@ void cnv_adj_destroy(cnv_adjoint* a) {
 
 int cnv_adj_create(const cnv_params* p, int checkpoint_every, double rtol_forward, double rtol_adjoint,
                    cnv_adjoint** out) {
+    if (!out) return afail(nullptr, CNV_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (!p || checkpoint_every < 1 || !(rtol_forward > 0) || !(rtol_adjoint > 0))
         return afail(nullptr, CNV_ERR_ARG, "adjoint: need params, checkpoint_every >= 1 and positive tolerances");

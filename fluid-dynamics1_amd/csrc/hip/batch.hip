@@ -570,6 +570,7 @@ void cnv_batch_destroy(cnv_batch* b) {
 }
 
 int cnv_batch_create(const cnv_params* items, int count, cnv_batch** out) {
+    if (!out) return bfail(nullptr, CNV_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (!items || count <= 0) return bfail(nullptr, CNV_ERR_ARG, "empty batch");
     const cnv_params& p0 = items[0];

@@ -158,6 +158,7 @@ static int upload_tables(cnv_ctx* c) {
 }
 
 int cnv_set_state(cnv_ctx* c, const double* w, const double* psi, const double* u, const double* v) {
+    if (!c) return cnv_fail(nullptr, CNV_ERR_ARG, "null context");
     const size_t bytes = sizeof(double) * c->cells;
     if (w) HIPC(c, hipMemcpyAsync(c->w[c->wcur], w, bytes, hipMemcpyHostToDevice, c->stream));
     if (psi) {
@@ -183,6 +184,7 @@ static int current_psi(cnv_ctx* c, const double** out) {
 }
 
 int cnv_get_state(cnv_ctx* c, double* w, double* psi, double* u, double* v) {
+    if (!c) return cnv_fail(nullptr, CNV_ERR_ARG, "null context");
     const size_t bytes = sizeof(double) * c->cells;
     if (psi) {
         const double* src = nullptr;
@@ -198,6 +200,7 @@ int cnv_get_state(cnv_ctx* c, double* w, double* psi, double* u, double* v) {
 }
 
 int cnv_device_state(cnv_ctx* c, double** w, double** psi, double** u, double** v) {
+    if (!c) return cnv_fail(nullptr, CNV_ERR_ARG, "null context");
     if (psi) {
         const double* src = nullptr;
         int rc = current_psi(c, &src);
@@ -250,6 +253,7 @@ void cnv_destroy(cnv_ctx* c) {
 }
 
 int cnv_create(const cnv_params* p, cnv_ctx** out) {
+    if (!out) return cnv_fail(nullptr, CNV_ERR_ARG, "null output pointer");
     *out = nullptr;
     if (!p) return cnv_fail(nullptr, CNV_ERR_ARG, "null params");
     if (p->n < 8) return cnv_fail(nullptr, CNV_ERR_ARG, "grid side n=%d: need n >= 8", p->n);
@@ -333,7 +337,7 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
     return CNV_OK;
 }
 
-int cnv_sweeps_per_launch(const cnv_ctx* c) { return c->shape.s; }
+int cnv_sweeps_per_launch(const cnv_ctx* c) { return c ? c->shape.s : 0; }
 
 // ---- SOR driver ---------------------------------------------------------------------------------
 static int poisson_fp64(cnv_ctx* c, const double* f_src, int negate, SorCtl* fin) {
@@ -663,6 +667,7 @@ int cnv_step_timed(cnv_ctx* c, int nsteps, cnv_step_stats* stats, double* elapse
 }
 
 int cnv_poisson_device(cnv_ctx* c, const double* d_f, double* d_psi, int* k, double* e) {
+    if (!c || !d_f || !d_psi) return cnv_fail(c, CNV_ERR_ARG, "bad arguments to cnv_poisson_device");
     if (hipSetDevice(c->p.device) != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipSetDevice failed");
     SorCtl fin{};
     int rc = poisson_fp64(c, d_f, 0, &fin);
@@ -677,6 +682,7 @@ int cnv_poisson_device(cnv_ctx* c, const double* d_f, double* d_psi, int* k, dou
 }
 
 int cnv_poisson_host(cnv_ctx* c, const double* f, double* psi, int* k, double* e) {
+    if (!c || !f || !psi) return cnv_fail(c, CNV_ERR_ARG, "bad arguments to cnv_poisson_host");
     if (hipSetDevice(c->p.device) != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipSetDevice failed");
     const size_t bytes = sizeof(double) * c->cells;
     // stage f in the spare w buffer: the solve reads it as its source term directly
@@ -794,7 +800,10 @@ int cnv_calibration_copy(cnv_ctx* c, long long nbytes, double* ms) {
     if (hipSetDevice(c->p.device) != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipSetDevice failed");
     double *a = nullptr, *b = nullptr;
     HIPC(c, hipMalloc(&a, nbytes));
-    HIPC(c, hipMalloc(&b, nbytes));
+    if (hipMalloc(&b, nbytes) != hipSuccess) {
+        (void)hipFree(a);
+        return cnv_fail(c, CNV_ERR_HIP, "hipMalloc of %lld bytes failed", nbytes);
+    }
     int rc = CNV_OK;
     if (copy_launch(b, a, nbytes / 8, c->stream)) rc = cnv_fail(c, CNV_ERR_HIP, "copy kernel failed");  // warm
     if (rc == CNV_OK && (hipEventRecord(c->t0, c->stream) != hipSuccess || copy_launch(b, a, nbytes / 8, c->stream) ||

@@ -116,7 +116,8 @@ int cnv_set_poisson_params(cnv_ctx *ctx, double dx, double dy, double beta, doub
                            int poisson_type);
 
 /* Poisson solve alone on this context's grid: psi = SOR(f) from zero (src/poisson.c:224-285),
- * f given as a host array; psi written to host.  k/e as logged by the reference. */
+ * f given as a host array; psi written to host.  k/e as logged by the reference.  Runs in the
+ * context's own buffers: the context's psi (cnv_get_state) becomes this solve's result. */
 int cnv_poisson_host(cnv_ctx *ctx, const double *f, double *psi, int *k, double *e);
 
 /* Same on device pointers (f, psi: n*n doubles in device memory), on the context stream. */

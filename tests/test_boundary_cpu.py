@@ -154,3 +154,40 @@ def test_params_layout_matches_library():
     import cnv
     L = cnv.lib()
     assert L.cnv_params_size() == ctypes.sizeof(cnv.Params)
+
+
+def test_device_api_argument_errors_without_gpu():
+    # argument checks come before any HIP call: each bad call returns CNV_ERR_ARG with a message
+    import ctypes
+    import threading
+    import cnv
+    L = cnv.lib()
+    p = cnv.default_params(32)
+    assert L.cnv_create(ctypes.byref(p), None) == cnv.ERR_ARG
+    assert b"null output pointer" in L.cnv_last_error(None)
+    h = ctypes.c_void_p()
+    bad = cnv.default_params(32, order=3)
+    assert L.cnv_create(ctypes.byref(bad), ctypes.byref(h)) == cnv.ERR_ARG and not h.value
+    assert b"valid orders are 2, 4 or 6" in L.cnv_last_error(None)
+    assert L.cnv_step(None, 1, None) == cnv.ERR_ARG
+    assert L.cnv_set_state(None, None, None, None, None) == cnv.ERR_ARG
+    assert L.cnv_get_state(None, None, None, None, None) == cnv.ERR_ARG
+    assert L.cnv_device_state(None, None, None, None, None) == cnv.ERR_ARG
+    assert L.cnv_poisson_device(None, None, None, None, None) == cnv.ERR_ARG
+    assert L.cnv_sweeps_per_launch(None) == 0
+    assert L.cnv_batch_create(ctypes.byref(p), 1, None) == cnv.ERR_ARG
+    assert L.cnv_adj_create(ctypes.byref(p), 10, 1e-12, 1e-12, None) == cnv.ERR_ARG
+    # the fallback message (no context) belongs to the calling thread
+    assert L.cnv_step(None, -1, None) == cnv.ERR_ARG
+    mine = L.cnv_last_error(None)
+    seen = []
+
+    def other():
+        L.cnv_create(ctypes.byref(bad), ctypes.byref(ctypes.c_void_p()))
+        seen.append(L.cnv_last_error(None))
+
+    t = threading.Thread(target=other)
+    t.start()
+    t.join()
+    assert b"valid orders" in seen[0]
+    assert L.cnv_last_error(None) == mine == b"bad arguments to cnv_step"
