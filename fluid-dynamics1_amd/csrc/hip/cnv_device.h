@@ -36,8 +36,6 @@ struct SorCtl {
 struct Diag {
     unsigned long long kmax;
     unsigned long long kmin;
-    double outer_norm;  // defect correction: L1 norm of the FP64 Jacobi-scaled residual
-    double pad;
 };
 
 __host__ __device__ inline unsigned long long dkey(double x) {
@@ -131,9 +129,8 @@ int add_launch(const double* a, const double* b, double* out, long long count, i
 int copy_launch(double* dst, const double* src, long long count, hipStream_t st);
 
 // defect correction
-int dc_residual_launch(const double* w, const double* psi, float* r32, Diag* diag, int n,
-                       int ld, double c1, double c2, double c3, double B, double beta,
-                       hipStream_t st);
+int dc_residual_launch(const double* w, const double* psi, float* r32, int n, int ld, double c1,
+                       double c2, double c3, double B, hipStream_t st);
 int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCtl* fin, int n,
                       int ld, hipStream_t st);
 // budget != nullptr: the inner solve's sweep budget is read from that device counter

@@ -365,7 +365,7 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) 
     *fin = SorCtl{};
     fin->status = SOR_FAILED;
     for (int o = 0; o < max_outer && total < c->p.itmax; ++o) {
-        LAUNCH(c, dc_residual_launch(w_src, c->psi[0], c->r32, c->diag, c->n, c->ld, c1, c2, c3, B, c->p.beta, c->stream));
+        LAUNCH(c, dc_residual_launch(w_src, c->psi[0], c->r32, c->n, c->ld, c1, c2, c3, B, c->stream));
         LAUNCH(c, dc_inner_init_launch(c->e32[0], c->ctl, c->n, c->ld, c->p.inner_rtol, c->p.tol, c->p.itmax - total,
                                        c->stream));
         SorCtl snap{};
@@ -450,8 +450,8 @@ static int body_tail(cnv_ctx* c, int src) { return body_chunk_of(c, src, cnv_ctx
 // with the remaining budget (device counter) ...
 static int body_resid(cnv_ctx* c, int src) {
     const double dx = c->p.dx, dy = c->p.dy;
-    LAUNCH(c, dc_residual_launch(c->w[src], c->psi[0], c->r32, c->diag, c->n, c->ld, dy * dy, dx * dx,
-                                 dx * dx * dy * dy, 2 * (dx * dx + dy * dy), c->p.beta, c->stream));
+    LAUNCH(c, dc_residual_launch(c->w[src], c->psi[0], c->r32, c->n, c->ld, dy * dy, dx * dx,
+                                 dx * dx * dy * dy, 2 * (dx * dx + dy * dy), c->stream));
     LAUNCH(c, dc_inner_init_launch(c->e32[0], c->ctl, c->n, c->ld, c->p.inner_rtol, c->p.tol, 0, c->stream,
                                    c->d_budget));
     return CNV_OK;
@@ -706,9 +706,9 @@ int cnv_time_sor(cnv_ctx* c, int launches, double* mean_ms) {
     const bool dc = c->p.precision == CNV_PREC_FP32_DC;
     // a solve that never converges (tol < 0) and never runs out of budget, on the current w
     if (dc) {
-        LAUNCH(c, dc_residual_launch(c->w[c->wcur], c->psi[0], c->r32, c->diag, c->n, c->ld, c->p.dy * c->p.dy,
+        LAUNCH(c, dc_residual_launch(c->w[c->wcur], c->psi[0], c->r32, c->n, c->ld, c->p.dy * c->p.dy,
                                      c->p.dx * c->p.dx, c->p.dx * c->p.dx * c->p.dy * c->p.dy,
-                                     2 * (c->p.dx * c->p.dx + c->p.dy * c->p.dy), c->p.beta, c->stream));
+                                     2 * (c->p.dx * c->p.dx + c->p.dy * c->p.dy), c->stream));
         LAUNCH(c, dc_inner_init_launch(c->e32[0], c->ctl, c->n, c->ld, 0.0, -1.0, 0x7fffffff, c->stream));
     } else {
         LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, -1.0, 0x7fffffff, c->stream));

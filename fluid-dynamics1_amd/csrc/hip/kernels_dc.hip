@@ -78,8 +78,8 @@ __global__ void __launch_bounds__(kBlock) transport_fp32_kernel(const double* __
 static inline int launched() { return hipGetLastError() == hipSuccess ? 0 : 1; }
 static inline unsigned blocks_for(long long count) { return (unsigned)((count + kBlock - 1) / kBlock); }
 
-int dc_residual_launch(const double* w, const double* psi, float* r32, Diag*, int n, int ld, double c1,
-                       double c2, double c3, double B, double, hipStream_t st) {
+int dc_residual_launch(const double* w, const double* psi, float* r32, int n, int ld, double c1,
+                       double c2, double c3, double B, hipStream_t st) {
     dim3 grid((n + 63) / 64, (n + 3) / 4);
     hipLaunchKernelGGL(dc_residual_kernel, grid, dim3(kBlock), 0, st, w, psi, r32, n, ld, c1, c2, c3, B);
     return launched();

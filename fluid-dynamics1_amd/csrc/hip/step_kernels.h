@@ -124,7 +124,6 @@ __device__ inline void ctl_reset(SorCtl* sl, double tol, double rtol, int itmax)
 __device__ inline void diag_reset(Diag* d) {
     d->kmax = dkey(-DBL_MAX);
     d->kmin = dkey(DBL_MAX);
-    d->outer_norm = 0.0;
 }
 
 // ---- velocities u = DY psi, v = -(DX psi) at (i, j) (src/main.c:366-383) -------------------

@@ -78,7 +78,8 @@ typedef struct {
     int k;                 /* logged Poisson index = sweeps - 1 (src/poisson.c:275)          */
     int sweeps;            /* SOR sweeps of this step (FP64) / inner FP32 sweeps (DC)        */
     int outer;             /* defect-correction iterations (DC), 0 for FP64                  */
-    double e;              /* L1 update of the last sweep (FP64) / outer norm (DC)           */
+    double e;              /* L1 update of the last sweep (FP64) / of the last tested inner
+                              FP32 sweep (DC)                                                 */
     double cont_max, cont_min; /* continuity diagnostic, if enabled                          */
 } cnv_step_stats;
 
