@@ -39,7 +39,7 @@ C_CELLS = N * N
 
 def workload_params(cnv, precision, args, **kw):
     # FP64 steps replay captured HIP graphs unless --eager; the refined form runs eager launches,
-    # which measured 0.7% faster (5% with the reference's inner relaxation): its step needs tens
+    # which measured 1.1% faster (6% with the reference's inner relaxation): its step needs tens
     # of graph launches (each outer iteration's residual, chunks and correction), each with its
     # own submission cost
     return cnv.default_params(N, Re=RE, dt=DT, order=6, beta=OMEGA, tol=TOL, itmax=50_000_000,
