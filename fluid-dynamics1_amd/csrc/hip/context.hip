@@ -90,7 +90,9 @@ const char* cnv_last_error(const cnv_ctx* c) { return c ? c->err : g_err; }
 int cnv_device_name(int device, char* buf, int len) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return cnv_fail(nullptr, CNV_ERR_NO_DEVICE, "no device %d", device);
-    snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+    // some drivers report no marketing name: name the ordinal instead
+    if (prop.name[0]) snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+    else snprintf(buf, len, "HIP device %d (%s)", device, prop.gcnArchName);
     return CNV_OK;
 }
 
