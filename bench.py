@@ -232,6 +232,8 @@ def main():
     ap.add_argument("--profile-sor", type=int, default=0, metavar="LAUNCHES",
                     help="profiling mode: one step, then LAUNCHES timed SOR launches and one 256 MiB "
                          "calibration copy (for counter scaling); prints no JSON")
+    ap.add_argument("--profile-form", choices=("fp64", "refined"), default="fp64",
+                    help="profiling mode: the FP64 step's SOR kernel, or the refined step's FP32 one")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the single cavity would shard "
@@ -246,8 +248,9 @@ def main():
           file=sys.stderr, flush=True)
 
     if args.profile_sor:
-        cav = cnv.Cavity(workload_params(cnv, cnv.PREC_FP64, args))
-        timed_steps(cav, 1, "fp64")
+        refined = args.profile_form == "refined"
+        cav = cnv.Cavity(workload_params(cnv, cnv.PREC_FP32_DC if refined else cnv.PREC_FP64, args))
+        timed_steps(cav, 1, "dc" if refined else "fp64")
         ms = cav.time_sor(args.profile_sor)
         cms = cav.calibration_copy(256 << 20)
         print("bench: profile mode: SOR launch %.4f ms, calibration copy of 256 MiB %.4f ms" % (ms, cms),
