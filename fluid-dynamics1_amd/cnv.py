@@ -69,6 +69,8 @@ def lib():
         L.cnv_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepStats)]
         L.cnv_step_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepStats), C.POINTER(C.c_double)]
         L.cnv_device_state.argtypes = [C.c_void_p] + [C.POINTER(C.c_void_p)] * 4
+        L.cnv_snapshot_begin.argtypes = [C.c_void_p]
+        L.cnv_snapshot_wait.argtypes = [C.c_void_p] + [C.POINTER(C.POINTER(C.c_double))] * 4
         L.cnv_poisson_host.argtypes = [C.c_void_p, _DP, _DP, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.cnv_poisson_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int),
                                          C.POINTER(C.c_double)]
@@ -174,6 +176,18 @@ class Cavity:
     def get_state(self):
         out = [np.empty((self.n, self.n)) for _ in range(4)]
         _check(lib().cnv_get_state(self.h, *[a.ctypes.data for a in out]), self.h)
+        return dict(zip(("w", "psi", "u", "v"), out))
+
+    def snapshot_begin(self):
+        """Queue an asynchronous copy of the state (taken after the steps already queued)."""
+        _check(lib().cnv_snapshot_begin(self.h), self.h)
+
+    def snapshot_wait(self):
+        """Wait for the last snapshot; returns copies of its fields."""
+        ptrs = [C.POINTER(C.c_double)() for _ in range(4)]
+        rc = lib().cnv_snapshot_wait(self.h, *[C.byref(p) for p in ptrs])
+        _check(rc, None)
+        out = [np.ctypeslib.as_array(p, shape=(self.n, self.n)).copy() for p in ptrs]
         return dict(zip(("w", "psi", "u", "v"), out))
 
     def step(self, nsteps=1, check=True):

@@ -127,6 +127,8 @@ int euler_launch(double* w, const double* dwdx, const double* dwdy, const double
 int add_launch(const double* a, const double* b, double* out, long long count, int sub,
                hipStream_t st);
 int copy_launch(double* dst, const double* src, long long count, hipStream_t st);
+int snapshot_launch(const double* w, const double* psi0, const double* psi1, const double* u, const double* v,
+                    const SorCtl* fin, int psi_fixed, double* dst, long long count, hipStream_t st);
 
 // defect correction
 int dc_residual_launch(const double* w, const double* psi, float* r32, int n, int ld, double c1,

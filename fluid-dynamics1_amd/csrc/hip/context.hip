@@ -201,6 +201,44 @@ int cnv_get_state(cnv_ctx* c, double* w, double* psi, double* u, double* v) {
     return CNV_OK;
 }
 
+int cnv_snapshot_begin(cnv_ctx* c) {
+    if (!c) return cnv_fail(nullptr, CNV_ERR_ARG, "null context");
+    if (hipSetDevice(c->p.device) != hipSuccess) return cnv_fail(c, CNV_ERR_HIP, "hipSetDevice failed");
+    const size_t bytes = sizeof(double) * c->cells * 4;
+    if (!c->d_snap) {  // first snapshot: its buffers, stream and events
+        HIPC(c, hipMalloc(&c->d_snap, bytes));
+        HIPC(c, hipHostMalloc(&c->h_snap, bytes, hipHostMallocDefault));
+        HIPC(c, hipStreamCreateWithFlags(&c->snap_stream, hipStreamNonBlocking));
+        HIPC(c, hipEventCreateWithFlags(&c->ev_snap_ready, hipEventDisableTiming));
+        HIPC(c, hipEventCreateWithFlags(&c->ev_snap_done, hipEventDisableTiming));
+    }
+    // the previous snapshot's transfer reads the device copy that is about to be overwritten
+    if (c->snap_begun) HIPC(c, hipEventSynchronize(c->ev_snap_done));
+    const bool fixed = c->p.precision == CNV_PREC_FP32_DC;  // the refined form's psi is psi[0]
+    LAUNCH(c, snapshot_launch(c->w[c->wcur], c->psi[0], c->psi[1], c->u, c->v, c->ctl + c->fin_slot, fixed,
+                              c->d_snap, (long long)c->cells, c->stream));
+    HIPC(c, hipEventRecord(c->ev_snap_ready, c->stream));
+    HIPC(c, hipStreamWaitEvent(c->snap_stream, c->ev_snap_ready, 0));
+    HIPC(c, hipMemcpyAsync(c->h_snap, c->d_snap, bytes, hipMemcpyDeviceToHost, c->snap_stream));
+    HIPC(c, hipEventRecord(c->ev_snap_done, c->snap_stream));
+    c->snap_begun = true;
+    return CNV_OK;
+}
+
+int cnv_snapshot_wait(cnv_ctx* c, const double** w, const double** psi, const double** u, const double** v) {
+    // may run on another thread while c steps: it touches only the snapshot's event and buffer,
+    // and reports failures through this thread's message only
+    if (!c || !c->snap_begun) return cnv_fail(nullptr, CNV_ERR_ARG, "no snapshot was begun on this context");
+    const hipError_t e = hipEventSynchronize(c->ev_snap_done);
+    if (e != hipSuccess) return cnv_fail(nullptr, CNV_ERR_HIP, "snapshot transfer failed: %s", hipGetErrorString(e));
+    const double* h = c->h_snap;
+    if (w) *w = h;
+    if (psi) *psi = h + c->cells;
+    if (u) *u = h + 2 * c->cells;
+    if (v) *v = h + 3 * c->cells;
+    return CNV_OK;
+}
+
 int cnv_device_state(cnv_ctx* c, double** w, double** psi, double** u, double** v) {
     if (!c) return cnv_fail(nullptr, CNV_ERR_ARG, "null context");
     if (psi) {
@@ -239,13 +277,18 @@ static void drop_graphs(cnv_ctx* c) {
 void cnv_destroy(cnv_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->snap_stream) (void)hipStreamSynchronize(c->snap_stream);
     drop_graphs(c);
-    void* dev[] = {c->w[0], c->w[1], c->psi[0], c->psi[1], c->u, c->v, c->d_start, c->d_tab,
-                   c->d_ftab, c->ctl, c->diag, c->partial, c->e32[0], c->e32[1], c->r32, c->d_budget};
+    void* dev[] = {c->w[0], c->w[1], c->psi[0], c->psi[1], c->u, c->v, c->d_start, c->d_tab, c->d_ftab,
+                   c->ctl, c->diag, c->partial, c->e32[0], c->e32[1], c->r32, c->d_budget, c->d_snap};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->h_diag) (void)hipHostFree(c->h_diag);
+    if (c->h_snap) (void)hipHostFree(c->h_snap);
+    if (c->ev_snap_ready) (void)hipEventDestroy(c->ev_snap_ready);
+    if (c->ev_snap_done) (void)hipEventDestroy(c->ev_snap_done);
+    if (c->snap_stream) (void)hipStreamDestroy(c->snap_stream);
     for (int s = 0; s < kSlots; ++s)
         if (c->ev[s]) (void)hipEventDestroy(c->ev[s]);
     if (c->t0) (void)hipEventDestroy(c->t0);

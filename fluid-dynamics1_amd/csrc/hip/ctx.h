@@ -61,6 +61,13 @@ struct cnv_ctx {
     hipGraphExec_t g_correct = nullptr;
     hipGraphExec_t g_post = nullptr;
     int* d_budget = nullptr;  // refined form, captured: the solve's remaining sweep budget
+    // output snapshots (cnv_snapshot_begin/wait): a device copy of (w, psi, u, v) taken in stream
+    // order, then transferred to pinned host memory on a stream of its own
+    double* d_snap = nullptr;
+    double* h_snap = nullptr;
+    hipStream_t snap_stream = nullptr;
+    hipEvent_t ev_snap_ready = nullptr, ev_snap_done = nullptr;
+    bool snap_begun = false;
     char err[256] = "";
 };
 

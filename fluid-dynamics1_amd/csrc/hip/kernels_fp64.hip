@@ -98,6 +98,24 @@ __global__ void __launch_bounds__(kBlock) copy_kernel(double* dst, const double*
     if (q < count) dst[q] = src[q];
 }
 
+// State snapshot for output: dst = [w | psi | u | v], each `count` doubles, with psi read from the
+// buffer that holds the solve's final iterate (fin->parity), or from psi0 when psi_fixed (the
+// refined form accumulates psi there).
+__global__ void __launch_bounds__(kBlock) snapshot_kernel(const double* __restrict__ w, const double* psi0,
+                                                         const double* psi1, const double* __restrict__ u,
+                                                         const double* __restrict__ v, const SorCtl* fin,
+                                                         int psi_fixed, double* __restrict__ dst,
+                                                         long long count) {
+    const double* __restrict__ psi = (psi_fixed || !(fin->parity & 1)) ? psi0 : psi1;
+    const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (q < count) {
+        dst[q] = w[q];
+        dst[count + q] = psi[q];
+        dst[2 * count + q] = u[q];
+        dst[3 * count + q] = v[q];
+    }
+}
+
 // ---- host wrappers -------------------------------------------------------------------------
 static inline int launched() { return hipGetLastError() == hipSuccess ? 0 : 1; }
 static inline unsigned blocks_for(long long count) { return (unsigned)((count + kBlock - 1) / kBlock); }
@@ -173,6 +191,13 @@ int add_launch(const double* a, const double* b, double* out, long long count, i
 
 int copy_launch(double* dst, const double* src, long long count, hipStream_t st) {
     hipLaunchKernelGGL(copy_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, st, dst, src, count);
+    return launched();
+}
+
+int snapshot_launch(const double* w, const double* psi0, const double* psi1, const double* u, const double* v,
+                    const SorCtl* fin, int psi_fixed, double* dst, long long count, hipStream_t st) {
+    hipLaunchKernelGGL(snapshot_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, st, w, psi0, psi1, u, v, fin,
+                       psi_fixed, dst, count);
     return launched();
 }
 

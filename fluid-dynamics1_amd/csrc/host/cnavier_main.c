@@ -10,6 +10,7 @@
  *   cnavier [config_file] [output_folder]      cnavier --help | --help-config
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -53,7 +54,8 @@ static void help_config(void)
     printf("            nsteps (default (int)(tf/dt-1)+1), precision (fp64 | fp32dc), inner_rtol (1e-5),\n");
     printf("            inner_beta (fp32dc inner relaxation: 0 = sor_omega, or \"reference\"),\n");
     printf("            sweeps_per_launch (0 = tuned default), diagnostics (1), device (0), use_graph (0),\n");
-    printf("            warm_start (0; 1 = each Poisson solve starts from the previous psi, not 0)\n");
+    printf("            warm_start (0; 1 = each Poisson solve starts from the previous psi, not 0),\n");
+    printf("            output_async (1 = VTK files written by a second thread while the next steps run)\n");
 }
 
 static mtrx to_mtrx(const double *a, int n)
@@ -61,6 +63,67 @@ static mtrx to_mtrx(const double *a, int n)
     mtrx A = initm(n, n);
     for (int i = 0; i < n; i++) memcpy(A.M[i], a + (size_t)i * n, sizeof(double) * n);
     return A;
+}
+
+/* The four VTK files of one output step, in the reference's order (src/main.c:429-436). */
+static void write_vtk_set(const double *w, const double *psi, const double *u, const double *v, int n,
+                          const char *dir)
+{
+    const double *order[4] = {psi, w, u, v};
+    char *titles[4] = {"stream-function", "vorticity", "x-velocity", "y-velocity"};
+    for (int q = 0; q < 4; q++) {
+        mtrx A = to_mtrx(order[q], n);
+        printvtk(A, titles[q], dir);
+        A.M = freem(A);
+    }
+}
+
+/* output_async: one snapshot's files, written on a thread of its own while the next steps run
+ * (the snapshot was queued by cnv_snapshot_begin; one writer at a time keeps the files' order) */
+typedef struct {
+    cnv_ctx *ctx;
+    int n;
+    const char *dir;
+    int rc;
+    pthread_t thread;
+    int running;
+} vtk_writer;
+
+static void *vtk_writer_main(void *arg)
+{
+    vtk_writer *wr = (vtk_writer *)arg;
+    const double *w, *psi, *u, *v;
+    wr->rc = cnv_snapshot_wait(wr->ctx, &w, &psi, &u, &v);
+    if (wr->rc == CNV_OK) write_vtk_set(w, psi, u, v, wr->n, wr->dir);
+    return NULL;
+}
+
+/* waits for the writer of the previous snapshot, if any; returns its status */
+static int vtk_writer_join(vtk_writer *wr)
+{
+    if (!wr->running) return CNV_OK;
+    pthread_join(wr->thread, NULL);
+    wr->running = 0;
+    if (wr->rc != CNV_OK) printf("Error: %s\n", cnv_last_error(NULL));
+    return wr->rc;
+}
+
+static int vtk_writer_start(vtk_writer *wr)
+{
+    int rc = vtk_writer_join(wr);
+    if (rc != CNV_OK) return rc;
+    rc = cnv_snapshot_begin(wr->ctx);
+    if (rc != CNV_OK) {
+        printf("Error: %s\n", cnv_last_error(wr->ctx));
+        return rc;
+    }
+    if (pthread_create(&wr->thread, NULL, vtk_writer_main, wr) == 0) {
+        wr->running = 1;
+        return CNV_OK;
+    }
+    vtk_writer_main(wr);  /* no thread available: write the files here */
+    if (wr->rc != CNV_OK) printf("Error: %s\n", cnv_last_error(NULL));
+    return wr->rc;
 }
 
 int main(int argc, char *argv[])
@@ -173,11 +236,15 @@ int main(int argc, char *argv[])
     }
 
     const size_t cells = (size_t)n * n;
-    double *host = (double *)malloc(sizeof(double) * cells * 4);
-    if (!host) {
-        printf("** Error: insufficient memory **");
-        return 1;
+    double *host = NULL;  /* synchronous output only */
+    if (!cx.output_async) {
+        host = (double *)malloc(sizeof(double) * cells * 4);
+        if (!host) {
+            printf("** Error: insufficient memory **");
+            return 1;
+        }
     }
+    vtk_writer writer = {ctx, n, output_dir, CNV_OK, 0, 0};
     const double t_sim = now_s();
     for (int t = 0; t < nsteps; t++) {
         const double t0 = now_s();
@@ -186,11 +253,13 @@ int main(int argc, char *argv[])
         if (rc == CNV_NOT_CONVERGED) {
             logf_(log_file, "Error: maximum number of iterations achieved for Poisson equation.\n");
             printf("Error: maximum number of iterations achieved for Poisson equation.\n");
+            vtk_writer_join(&writer);
             cnv_destroy(ctx);
             return 1;
         }
         if (rc != CNV_OK) {
             printf("Error: %s\n", cnv_last_error(ctx));
+            vtk_writer_join(&writer);
             cnv_destroy(ctx);
             return 1;
         }
@@ -210,22 +279,27 @@ int main(int argc, char *argv[])
         else
             logf_(log_file, "Est. remaining: -- s\n");
         if (cfg.output_interval > 0 && t % cfg.output_interval == 0) {
-            double *w = host, *psi = host + cells, *u = host + 2 * cells, *v = host + 3 * cells;
-            if (cnv_get_state(ctx, w, psi, u, v) != CNV_OK) {
-                printf("Error: %s\n", cnv_last_error(ctx));
-                return 1;
-            }
-            const double *order[4] = {psi, w, u, v};
-            char *titles[4] = {"stream-function", "vorticity", "x-velocity", "y-velocity"};
-            for (int q = 0; q < 4; q++) {  /* src/main.c:429-436 */
-                mtrx A = to_mtrx(order[q], n);
-                printvtk(A, titles[q], output_dir);
-                A.M = freem(A);
+            if (cx.output_async) {
+                if (vtk_writer_start(&writer) != CNV_OK) {
+                    vtk_writer_join(&writer);
+                    cnv_destroy(ctx);
+                    return 1;
+                }
+            } else {
+                double *w = host, *psi = host + cells, *u = host + 2 * cells, *v = host + 3 * cells;
+                if (cnv_get_state(ctx, w, psi, u, v) != CNV_OK) {
+                    printf("Error: %s\n", cnv_last_error(ctx));
+                    cnv_destroy(ctx);
+                    return 1;
+                }
+                write_vtk_set(w, psi, u, v, n, output_dir);
             }
         }
     }
+    const int wrc = vtk_writer_join(&writer);
     cnv_destroy(ctx);
     free(host);
+    if (wrc != CNV_OK) return 1;
     printf("Simulation complete!\n");
     const double t_end = now_s();
     const double total_sim = t_end - t_sim;

@@ -107,6 +107,19 @@ int cnv_step(cnv_ctx *ctx, int nsteps, cnv_step_stats *stats);
 /* Timed variant: wall time of the step loop measured with HIP events on the context stream. */
 int cnv_step_timed(cnv_ctx *ctx, int nsteps, cnv_step_stats *stats, double *elapsed_ms);
 
+/* Output without stopping the GPU (no counterpart in the reference, which writes its VTK files
+ * between steps from host memory).  cnv_snapshot_begin queues, behind the steps already queued,
+ * a device copy of the state (w, psi, u, v) and its transfer to pinned host memory owned by the
+ * context, on a stream of its own, and returns at once: the steps that follow run while it
+ * transfers.  cnv_snapshot_wait blocks until that transfer is complete and returns host pointers
+ * to n*n doubles each (any may be NULL), valid until the next cnv_snapshot_begin.  It may be
+ * called from another thread while the context steps (its failures are reported in that
+ * thread's cnv_last_error(NULL)); cnv_snapshot_begin and everything else stay on the context's
+ * thread. */
+int cnv_snapshot_begin(cnv_ctx *ctx);
+int cnv_snapshot_wait(cnv_ctx *ctx, const double **w, const double **psi, const double **u,
+                      const double **v);
+
 /* Device pointers of the state (w, psi, u, v) for zero-copy callers; valid until the next step. */
 int cnv_device_state(cnv_ctx *ctx, double **w, double **psi, double **u, double **v);
 

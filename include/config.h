@@ -44,6 +44,8 @@ typedef struct {
     double inner_beta;      /* inner_beta: relaxation of the inner FP32 solves of fp32dc (0 =
                                sor_omega; "reference" = the reference's formula for nx, ny)     */
     int warm_start;         /* warm_start: 1 = each Poisson solve starts from the previous psi  */
+    int output_async;       /* output_async: 1 = VTK files written by a second thread while the
+                               next steps run (default); 0 = between steps, as the reference     */
     int has_sor_omega;
 } ConfigExt;
 

@@ -28,6 +28,10 @@ def test_every_declared_symbol_is_exported():
     missing = sorted(n for n in decl if n not in exported and n not in DECLARED_ONLY)
     assert not missing, missing
     assert "pressure" not in exported
+    # and nothing else: the kernels' C++ symbols and the host helpers stay local
+    every = {line.split()[-1] for line in out.splitlines() if len(line.split()) == 3}
+    extra = sorted(every - set(decl))
+    assert not extra, extra[:10]
 
 
 def test_reference_headers_signatures_present():
