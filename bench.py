@@ -12,7 +12,8 @@ solver stream), inputs resident in HBM.  roofline: the SOR kernel (the dominant 
 bytes per launch = sweeps_per_launch * 24 B * cells (read psi, read omega, write psi per sweep)
 over its mean launch duration, measured with HIP events on the stream it runs on.
 cpu_baseline: the oracle (the repository's CPU restatement of the reference step), OpenMP
-red-black, on a bounded sample of the same workload (see "sample").
+red-black, on a bounded sample of the same workload (see "sample"); serial_lexicographic: the
+same on one thread in the reference's default (serial, lexicographic) order.
 """
 import argparse
 import json
@@ -70,14 +71,17 @@ def read_traffic(name):
     return None
 
 
-def cpu_baseline(sample_seconds=25.0):
+def cpu_baseline(sample_seconds=25.0, serial=False):
     """Oracle at the same workload on the host cores: the first time step's transport plus a
-    bounded number of red-black SOR sweeps, timed; scaled to cell-steps/s with the GPU run's
-    mean sweeps per step (identical arithmetic gives identical sweep counts)."""
+    bounded number of SOR sweeps, timed; scaled to cell-steps/s with the GPU run's mean sweeps per
+    step.  Red-black on up to 16 threads (the reference's OpenMP build: identical arithmetic, so
+    identical sweep counts), or serial=True: one thread in lexicographic order (the reference's
+    default build; both orderings are consistent orderings of the 5-point operator, so SOR
+    converges at the same rate and the sweep counts nearly agree)."""
     from oracle import orc
-    threads = min(16, os.cpu_count() or 1)
+    threads = 1 if serial else min(16, os.cpu_count() or 1)
     p = orc.make_params(N, Re=RE, dt=DT, order=6, tol=-1.0, itmax=1, beta_=OMEGA,
-                        ordering=orc.REDBLACK, threads=threads)
+                        ordering=orc.LEX if serial else orc.REDBLACK, threads=threads)
     s = orc.State(p)
     orc.lib().orc_apply_velocity_bc(orc.C.byref(p), s.u, s.v)
     orc.lib().orc_wall_vorticity(orc.C.byref(p), s.u, s.v, s.w)
@@ -321,6 +325,13 @@ def main():
                "sample": "oracle (C restatement, OpenMP red-black, %d threads) at 1024^2: step-0 transport "
                          "(%.3f s) + %d SOR sweeps (%.2f ms/sweep), scaled by the GPU run's mean %.0f sweeps/step"
                          % (b["threads"], b["t_transport"], b["sweeps"], b["t_sweep"] * 1e3, kmean)}
+        bs = cpu_baseline(sample_seconds=8.0, serial=True)
+        t_step_s = bs["t_transport"] + kmean * bs["t_sweep"]
+        cpu["serial_lexicographic"] = {
+            "value": C / t_step_s / 1e6, "unit": "Mcell-steps/s", "cores": 1,
+            "sample": "oracle, one thread, lexicographic SOR (the reference's default build) at 1024^2: "
+                      "step-0 transport (%.3f s) + %d SOR sweeps (%.2f ms/sweep), scaled by the GPU run's "
+                      "mean %.0f sweeps/step" % (bs["t_transport"], bs["sweeps"], bs["t_sweep"] * 1e3, kmean)}
 
     traffic = read_traffic("sor_fp64_n1024_traffic.txt")
     line = {
