@@ -5,7 +5,7 @@ Bar: the plain FP64 path is bit-exact (up to the sign of zero) against the refer
 ordering -- fields, per-step Poisson index k and continuity max/min.  The L1 update e is summed
 in a different order on the GPU, so it is compared to 1e-12 relative.  Against the serial
 (lexicographic) reference the fixed point is the same, so with both solves driven to a tight
-tolerance the fields agree to 1e-9 relative L-infinity (the north-star parity bar).
+tolerance the fields agree to 1e-9 relative L-infinity (the parity bar across the two orderings).
 """
 import glob
 import os
