@@ -25,7 +25,7 @@ def run(tmp_path, text, name):
 
 def test_config_files_present():
     names = [os.path.basename(p) for p in CONFIGS]
-    assert names == ["config0_cpu_128.txt", "config1_gpu_1024.txt", "config2_refined_1024.txt"]
+    assert names == ["cavity1024_re1000_fp64.txt", "cavity1024_re1000_refined.txt", "cavity128_re100_omega1.8.txt"]
 
 
 @pytest.mark.parametrize("path", CONFIGS, ids=lambda p: os.path.basename(p)[:-4])
@@ -38,7 +38,7 @@ def test_config_runs_two_steps(tmp_path, path):
 
 
 def test_inner_beta_reference_key(tmp_path):
-    base = open(os.path.join(ROOT, "configs", "config2_refined_1024.txt")).read()
+    base = open(os.path.join(ROOT, "configs", "cavity1024_re1000_refined.txt")).read()
     base = base.replace("nx = 1024", "nx = 256").replace("ny = 1024", "ny = 256")
     base += "\nnsteps = 2\noutput_interval = 1000000\npoisson_tol = 1e-9\n"
     plain = [int(k) for k in SOLVED.findall(run(tmp_path, base, "plain"))]
