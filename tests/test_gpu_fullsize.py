@@ -80,8 +80,8 @@ def test_fullsize_step_bit_exact_reference_beta():
 
 
 def test_fullsize_metric_step_bit_exact():
-    # the metric's configuration exactly (omega = 1.95): the first step from rest, 48221 sweeps
-    # (about 9.6k launches of 5), against the oracle's OpenMP red-black step (about 35 s on 16
+    # the metric's configuration exactly (omega = 1.95): the first step from rest, 46352 sweeps
+    # (about 9.3k launches of 5), against the oracle's OpenMP red-black step (about 35 s on 16
     # host threads), bit for bit
     import os
     kw = dict(Re=RE, dt=DT, order=6, tol=TOL, itmax=50_000_000)
@@ -113,3 +113,32 @@ def test_large_grid_poisson_iterates_bit_exact(n):
     # the last sweep's L1 update, summed over thousands of per-workgroup partials (several per
     # thread in the decision's reduction): equal up to the summation order
     assert abs(e - st_or.e) <= 1e-12 * st_or.e, (e, st_or.e)
+
+
+def test_fullsize_benchmark_steps_match_oracle_digests():
+    # bench.py's warm-up step and the four steps it times (1024^2, omega = 1.95, tol 1e-10,
+    # graph-replayed: 46352, 48221, 49309, 50077 and 50668 sweeps), each equal to the oracle's
+    # step in k, e (to 1e-12: another summation order), continuity and every field, through
+    # SHA-256 digests (tests/golden/metric_steps_1024.npz, written by
+    # tests/golden/make_metric_steps.py)
+    import hashlib
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "metric_steps_1024.npz"))
+    n, Re, dt, order, omega, tol, itmax, steps = g["params"]
+    print("1024^2 benchmark steps against the oracle's digests", flush=True)
+
+    def digest(a):
+        return hashlib.sha256(np.ascontiguousarray(a + 0.0, dtype="<f8").tobytes()).digest()
+
+    p = cnv.default_params(int(n), Re=float(Re), dt=float(dt), order=int(order), beta=float(omega),
+                           tol=float(tol), itmax=int(itmax), use_graph=1)
+    with cnv.Cavity(p) as cav:
+        for t in range(int(steps)):
+            st = cav.step(1)[0]
+            s = cav.get_state()
+            print("step %d: k %d (oracle %d)" % (t, st.k, g["k"][t]), flush=True)
+            assert st.k == g["k"][t]
+            assert abs(st.e - g["e"][t]) <= 1e-12 * g["e"][t]
+            assert st.cont_max == g["cont_max"][t] and st.cont_min == g["cont_min"][t]
+            for i, f in enumerate(("w", "psi", "u", "v")):
+                assert digest(s[f]) == bytes(g["digests"][t][i]), (t, f)
