@@ -6,7 +6,7 @@
 
 #include "linearalg.h"
 
-mtrx Diff1(int n, int o, double dx); /* first derivative, order o */
-mtrx Diff2(int n, int o, double dx); /* second derivative, order o */
+mtrx Diff1(int n, int o, double dx); /* first derivative, order o (src/finitediff.c:51) */
+mtrx Diff2(int n, int o, double dx); /* second derivative, order o (src/finitediff.c:178) */
 
 #endif

@@ -18,10 +18,15 @@
 #define PI 3.14159265359
 
 double error(mtrx u1, mtrx u2); /* sum of |u2 - u1| in row-major order (src/poisson.c:34-60) */
+/* Gauss-Seidel (src/poisson.c:62) */
 mtrx poisson(mtrx f, double dx, double dy, int itmax, double tol);
+/* SOR with relaxation beta (src/poisson.c:111) */
 mtrx poisson_SOR(mtrx f, double dx, double dy, int itmax, double tol, double beta);
+/* Gauss-Seidel logging its sweep count to log_file, which may be NULL (src/poisson.c:176) */
 mtrx poisson_log(mtrx f, double dx, double dy, int itmax, double tol, FILE *log_file);
+/* SOR logging its sweep count (src/poisson.c:224) */
 mtrx poisson_SOR_log(mtrx f, double dx, double dy, int itmax, double tol, double beta, FILE *log_file);
-void set_poisson_openmp_config(int enabled); /* accepted for compatibility; the GPU path is used */
+/* accepted for compatibility; the GPU path is used (src/poisson.c:15) */
+void set_poisson_openmp_config(int enabled);
 
 #endif
