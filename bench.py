@@ -175,10 +175,11 @@ def adjoint_leg(cnv, steps, K):
     }
 
 
-def dc_leg(cnv, args, state64, ms64, spl, inner_beta, tag, time_sor=True):
+def dc_leg(cnv, args, state64, ms64, spl, inner_beta, tag, time_sor=True, transport_fp32=0):
     """The FP32 + FP64 defect-correction form of the same steps, compared with the FP64 run."""
     p = workload_params(cnv, cnv.PREC_FP32_DC, args)
     p.inner_beta = inner_beta
+    p.transport_fp32 = transport_fp32
     cd = cnv.Cavity(p)
     if args.warmup:
         timed_steps(cd, args.warmup, tag + " warmup")
@@ -196,6 +197,7 @@ def dc_leg(cnv, args, state64, ms64, spl, inner_beta, tag, time_sor=True):
 
     out = {
         "inner_beta": inner_beta if inner_beta > 0 else OMEGA,
+        "transport": "fp32" if transport_fp32 else "fp64",
         "value": C_CELLS * args.steps / (ms32 / 1e3) / 1e6,
         "ms_per_step": ms32 / args.steps,
         "speedup_vs_fp64": ms64 / ms32,
@@ -303,6 +305,10 @@ def main():
         # (src/main.c:134) instead of omega = 1.95: an algorithmic choice open to the inner solver
         dc["reference_inner_beta"] = dc_leg(cnv, args, state64, ms, spl, cnv.reference_beta(N), "dc-refbeta",
                                             time_sor=False)
+        # FP32 transport stencils as well: no measurable speed-up (transport is about 25 us of a
+        # step), and their rounding accumulates in omega step after step, so it is not the default
+        dc["transport_fp32"] = dc_leg(cnv, args, state64, ms, spl, 0.0, "dc-transport32", time_sor=False,
+                                      transport_fp32=1)
 
     # ---- batched 64-cavity workload -----------------------------------------------------------
     batched = None

@@ -229,6 +229,7 @@ thread_local char g_berr[256] = "";  // last error of this thread
 struct cnv_batch {
     int count = 0;
     int dc = 0, gs = 0;
+    bool transport_fp32 = false;  // refined form: FP32 transport stencils (default FP64)
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<cnv_ctx*> ctx;
@@ -445,7 +446,7 @@ static int batch_one_step(cnv_batch* b, cnv_step_stats* stats) {
     hipLaunchKernelGGL(batch_wall_vorticity_kernel, dim3(b->nb_edge), dim3(kBlock), 0, b->stream, b->d_items,
                        b->d_map_edge, win);
     BLAUNCH(b);
-    if (b->dc)
+    if (b->dc && b->transport_fp32)
         hipLaunchKernelGGL(batch_transport_kernel<true>, dim3(b->nb_cell), dim3(kBlock), 0, b->stream, b->d_items,
                            b->d_map_cell, win);
     else
@@ -578,9 +579,10 @@ int cnv_batch_create(const cnv_params* items, int count, cnv_batch** out) {
         const cnv_params& p = items[i];
         if (p.precision != p0.precision || p.order != p0.order || p.poisson_type != p0.poisson_type ||
             p.device != p0.device || p.sweeps_per_launch != p0.sweeps_per_launch ||
-            p.tile_waves != p0.tile_waves || p.wave_rows != p0.wave_rows)
+            p.tile_waves != p0.tile_waves || p.wave_rows != p0.wave_rows || p.transport_fp32 != p0.transport_fp32)
             return bfail(nullptr, CNV_ERR_ARG,
-                         "batch item %d: precision, order, poisson_type, device and launch shape must match item 0", i);
+                         "batch item %d: precision, order, poisson_type, device, launch shape and transport_fp32 "
+                         "must match item 0", i);
     }
     for (int i = 0; i < count; ++i)
         if (items[i].warm_start)
@@ -588,6 +590,7 @@ int cnv_batch_create(const cnv_params* items, int count, cnv_batch** out) {
     cnv_batch* b = new cnv_batch();
     b->count = count;
     b->dc = p0.precision == CNV_PREC_FP32_DC;
+    b->transport_fp32 = p0.transport_fp32 != 0;
     b->gs = p0.poisson_type == 1;
     b->device = p0.device;
     b->rtol = p0.inner_rtol;

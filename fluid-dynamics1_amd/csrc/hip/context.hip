@@ -458,8 +458,12 @@ static int body_pre(cnv_ctx* c, int wc) {
     const int n = c->n, ld = c->ld;
     LAUNCH(c, wall_vorticity_launch(c->w[wc], c->u, c->v, n, ld, c->st, c->bc, c->stream));
     if (is_dc(c)) {
-        LAUNCH(c, transport_fp32_launch(c->w[wc], c->w[wc ^ 1], c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re,
-                                        c->p.dt, c->stream));
+        if (c->p.transport_fp32)
+            LAUNCH(c, transport_fp32_launch(c->w[wc], c->w[wc ^ 1], c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re,
+                                            c->p.dt, c->stream));
+        else
+            LAUNCH(c, transport_fp64_launch(c->w[wc], c->w[wc ^ 1], c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re,
+                                            c->p.dt, c->stream));
         // psi[0] is the FP64 accumulator (a warm start keeps it); the solve's sweep budget
         LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, n, ld, c->p.tol, c->p.itmax, c->stream,
                                       !c->p.warm_start));
@@ -649,7 +653,7 @@ static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
     double* w_out = c->w[c->wcur ^ 1];
     const bool dc = c->p.precision == CNV_PREC_FP32_DC;
     LAUNCH(c, wall_vorticity_launch(w_in, c->u, c->v, n, ld, c->st, c->bc, c->stream));
-    if (dc)
+    if (dc && c->p.transport_fp32)
         LAUNCH(c, transport_fp32_launch(w_in, w_out, c->u, c->v, n, ld, c->st, c->bc, rRe, c->p.dt, c->stream));
     else
         LAUNCH(c, transport_fp64_launch(w_in, w_out, c->u, c->v, n, ld, c->st, c->bc, rRe, c->p.dt, c->stream));

@@ -55,7 +55,8 @@ static void help_config(void)
     printf("            inner_beta (fp32dc inner relaxation: 0 = sor_omega, or \"reference\"),\n");
     printf("            sweeps_per_launch (0 = tuned default), diagnostics (1), device (0), use_graph (0),\n");
     printf("            warm_start (0; 1 = each Poisson solve starts from the previous psi, not 0),\n");
-    printf("            output_async (1 = VTK files written by a second thread while the next steps run)\n");
+    printf("            output_async (1 = VTK files written by a second thread while the next steps run),\n");
+    printf("            transport_fp32 (0; 1 = fp32dc runs the transport stencils in FP32 as well)\n");
 }
 
 static mtrx to_mtrx(const double *a, int n)
@@ -200,6 +201,7 @@ int main(int argc, char *argv[])
     p.diagnostics = cx.diagnostics;
     p.use_graph = cx.use_graph;
     p.warm_start = cx.warm_start;
+    p.transport_fp32 = cx.transport_fp32;
     cnv_ctx *ctx = NULL;
     int rc = cnv_create(&p, &ctx);
     if (rc != CNV_OK) {

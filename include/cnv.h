@@ -71,6 +71,11 @@ typedef struct {
                               instead of 0 (src/poisson.c:229 starts from 0): fewer sweeps,
                               results that differ within the tolerance; off for parity.
                               One-cavity contexts only (a batch refuses it)                  */
+    int transport_fp32;    /* FP32_DC: 1 = the transport stencils in FP32 too.  Default 0: FP64
+                              transport (about 25 us of a 77 ms step at 1024^2); with FP32
+                              stencils the refined trajectory drifts from the FP64 one by their
+                              rounding, step after step (2e-6 relative after 600 steps at 256^2,
+                              Re 1000), against 1e-10 with FP64 transport                     */
 } cnv_params;
 
 typedef struct {

@@ -105,3 +105,40 @@ def test_dc_not_converged_and_deterministic():
     a = run(q, 3)[1]
     b = run(q, 3)[1]
     assert all(np.array_equal(a[f], b[f]) for f in ("psi", "w", "u", "v"))
+
+
+def dc_error_trajectory(n, steps, every, **extra):
+    kw = dict(Re=1000.0, dt=5e-4, order=6, tol=1e-10, itmax=5_000_000, beta=cnv.reference_beta(n))
+    errs = []
+    with cnv.Cavity(cnv.default_params(n, **kw)) as c64, \
+            cnv.Cavity(cnv.default_params(n, precision=cnv.PREC_FP32_DC, **kw, **extra)) as c32:
+        for t in range(0, steps, every):
+            c64.step(every)
+            c32.step(every)
+            s64, s32 = c64.get_state(), c32.get_state()
+            mid = n // 2
+            e = max(rel(s32["psi"], s64["psi"]), rel(s32["w"], s64["w"]),
+                    rel(s32["u"][:, mid], s64["u"][:, mid]), rel(s32["v"][mid, :], s64["v"][mid, :]))
+            errs.append(e)
+            print("  step %d: %.2e" % (t + every, e), flush=True)
+    return errs
+
+
+def test_dc_error_stays_bounded_over_many_steps():
+    # 600 steps at 256^2 (Re 1000, the reference's relaxation rule, tol 1e-10): with the transport
+    # stencils in FP64 the refined trajectory differs from the FP64 one only by the solver
+    # tolerance's own effect, about 1e-10, and does not drift (sampled every 100 steps)
+    print("dc over 600 steps at 256^2", flush=True)
+    errs = dc_error_trajectory(256, 600, 100)
+    assert max(errs) <= 1e-8, errs
+    assert errs[-1] <= 2.0 * errs[0], errs
+
+
+def test_dc_fp32_transport_option():
+    # transport_fp32 = 1 moves the transport stencils to FP32 as well: each step then adds an
+    # FP32-rounding-sized change to omega and the difference from the FP64 trajectory grows about
+    # linearly, near 2e-7 per 100 steps here; short runs stay within 1e-6
+    print("dc with FP32 transport, 200 steps at 256^2", flush=True)
+    errs = dc_error_trajectory(256, 200, 100, transport_fp32=1)
+    assert 1e-8 < errs[0] <= 1e-6 and errs[1] <= 1e-6, errs
+    assert errs[1] > errs[0], errs
