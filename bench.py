@@ -13,7 +13,9 @@ bytes per launch = sweeps_per_launch * 24 B * cells (read psi, read omega, write
 over its mean launch duration, measured with HIP events on the stream it runs on.
 cpu_baseline: the oracle (the repository's CPU restatement of the reference step), OpenMP
 red-black, on a bounded sample of the same workload (see "sample"); serial_lexicographic: the
-same on one thread in the reference's default (serial, lexicographic) order.
+same on one thread in the reference's default (serial, lexicographic) order; reference_build
+(when oracle/_ref is built): the reference's own OpenMP poisson_SOR_log, compiled from its
+sources, timed on a bounded number of sweeps of the same source.
 """
 import argparse
 import json
@@ -99,7 +101,7 @@ def cpu_baseline(sample_seconds=25.0, serial=False):
     t0 = time.perf_counter()
     orc.poisson(p, f)
     t_sweep = (time.perf_counter() - t0) / sweeps
-    return dict(t_transport=t_transport, t_sweep=t_sweep, sweeps=sweeps, threads=threads)
+    return dict(t_transport=t_transport, t_sweep=t_sweep, sweeps=sweeps, threads=threads, f=f)
 
 
 def batched_leg(cnv, steps, precision, tag):
@@ -338,6 +340,22 @@ def main():
             "sample": "oracle, one thread, lexicographic SOR (the reference's default build) at 1024^2: "
                       "step-0 transport (%.3f s) + %d SOR sweeps (%.2f ms/sweep), scaled by the GPU run's "
                       "mean %.0f sweeps/step" % (bs["t_transport"], bs["sweeps"], bs["t_sweep"] * 1e3, kmean)}
+        from oracle import ref_sor_sample
+        if ref_sor_sample.available():
+            # the reference's own poisson_SOR_log, compiled from its sources (oracle/_ref); its
+            # transport is not timed: at 1024^2 its dense Kronecker operators would need 8 TB
+            t_ref, n_ref = ref_sor_sample.sample(b["f"], OMEGA, b["threads"], sample_seconds=10.0)
+            t_step_r = b["t_transport"] + kmean * t_ref
+            cpu["reference_build"] = {
+                "value": C / t_step_r / 1e6, "unit": "Mcell-steps/s", "cores": b["threads"], "kind": "reference",
+                "sample": "the reference's own poisson_SOR_log (src/poisson.c, built by oracle/Makefile with its "
+                          "Makefile's flags -g -fopenmp and -DOPENMP_ENABLED: red-black, %d threads) at 1024^2: "
+                          "%d sweeps (%.2f ms/sweep) on the step-0 source, plus the oracle's step-0 transport "
+                          "(%.3f s; the reference's dense Kronecker transport cannot be formed at 1024^2), "
+                          "scaled by the GPU run's mean %.0f sweeps/step"
+                          % (b["threads"], n_ref, t_ref * 1e3, b["t_transport"], kmean)}
+        else:
+            print("bench: oracle/_ref not built; reference_build baseline skipped", file=sys.stderr, flush=True)
 
     traffic = read_traffic("sor_fp64_n1024_traffic.txt")
     line = {
