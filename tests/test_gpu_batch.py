@@ -43,6 +43,17 @@ def test_batch_equals_single(precision, steps, shape):
     # cavity's size): the reported L1 update is summed per tile
     ps = mix_params(precision, sweeps_per_launch=shape[0], tile_waves=shape[1], wave_rows=shape[2])
     print("batch of", [p.n for p in ps], "precision", precision, "shape", shape, flush=True)
+    check_batch_equals_single(ps, steps)
+
+
+def test_batch_fp32_transport_equals_single():
+    # the refined form with transport_fp32 = 1: the batched transport kernel's FP32 instantiation
+    ps = mix_params(cnv.PREC_FP32_DC, transport_fp32=1, sweeps_per_launch=5, tile_waves=16, wave_rows=4)
+    print("batch of", [p.n for p in ps], "refined, FP32 transport", flush=True)
+    check_batch_equals_single(ps, 2)
+
+
+def check_batch_equals_single(ps, steps):
     with cnv.Batch(ps) as b:
         bst = b.step(steps)
         bstate = [b.get_state(i) for i in range(len(ps))]
