@@ -104,6 +104,39 @@ def cpu_baseline(sample_seconds=25.0, serial=False):
     return dict(t_transport=t_transport, t_sweep=t_sweep, sweeps=sweeps, threads=threads, f=f)
 
 
+def host_buffer_leg(cnv, cav, state, params):
+    """One Poisson solve of the workload's source (f = -omega after the timed steps) through the
+    reference-compatible boundary, which hands over host arrays (cnv_poisson_host: f copied in, psi
+    copied out, pageable memory), beside the same solve on device buffers (cnv_poisson_device; the
+    buffers are a second context's w (holding f) and u (receiving psi)).  Both wall times include
+    the host's launch and poll work; their difference is the PCIe part."""
+    import ctypes
+    f = np.ascontiguousarray(-state["w"])
+    t0 = time.perf_counter()
+    rc, psi, k, e = cav.poisson(f)
+    t_host = time.perf_counter() - t0
+    if rc != cnv.OK:
+        raise RuntimeError("host-buffer Poisson solve failed (%d)" % rc)
+    with cnv.Cavity(params) as aux:
+        aux.set_state(w=f)
+        dw, du = ctypes.c_void_p(), ctypes.c_void_p()
+        cnv._check(cnv.lib().cnv_device_state(aux.h, ctypes.byref(dw), None, ctypes.byref(du), None), aux.h)
+        kd, ed = ctypes.c_int(), ctypes.c_double()
+        t0 = time.perf_counter()
+        rc = cnv.lib().cnv_poisson_device(cav.h, dw, du, ctypes.byref(kd), ctypes.byref(ed))
+        t_dev = time.perf_counter() - t0
+        if rc != cnv.OK:
+            raise RuntimeError("device-buffer Poisson solve failed (%d)" % rc)
+        same = bool(np.array_equal(aux.get_state()["u"], psi)) and (k, e) == (kd.value, ed.value)
+    print("bench: Poisson solve host buffers %.2f ms, device buffers %.2f ms, %d sweeps, identical %s"
+          % (t_host * 1e3, t_dev * 1e3, k + 1, same), file=sys.stderr, flush=True)
+    return {"solve_ms_host_buffers": t_host * 1e3, "solve_ms_device_buffers": t_dev * 1e3,
+            "pcie_share": (t_host - t_dev) / t_host, "sweeps": k + 1, "identical_results": same,
+            "Mcells_per_s_host_buffers": N * N / t_host / 1e6,
+            "note": "one SOR solve of f = -omega (1024^2, omega 1.95, tol 1e-10) through cnv_poisson_host "
+                    "(8 MiB in, 8 MiB out, pageable) and through cnv_poisson_device; not the metric"}
+
+
 def batched_leg(cnv, steps, precision, tag):
     """The 64-cavity workload (seed 0) in one batch: autotuned launch shape, one warmup step, then
     `steps` timed steps; beside it the same cavities stepped one at a time through the one-cavity
@@ -234,6 +267,8 @@ def main():
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start FP64 leg")
     ap.add_argument("--sor-launches", type=int, default=400)
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-cavity batched leg")
+    ap.add_argument("--no-host-io", action="store_true",
+                    help="skip the Poisson solve through host buffers (the PCIe-inclusive rate)")
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 256^2 adjoint leg")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
@@ -279,6 +314,8 @@ def main():
     spl = cav.sweeps_per_launch
     print("bench: fp64 %.3f ms/step, sweeps/step %s" % (ms / args.steps, sweeps), file=sys.stderr, flush=True)
     sor_ms = cav.time_sor(args.sor_launches)
+    host_io = None if args.no_host_io else host_buffer_leg(
+        cnv, cav, state64, workload_params(cnv, cnv.PREC_FP64, args))
     cav.close()
     achieved = spl * BYTES_PER_CELL_SWEEP * C / (sor_ms / 1e3)
 
@@ -382,6 +419,7 @@ def main():
         "cpu_baseline": cpu,
         "fp64_warm_start": warm,
         "fp32_refined": dc,
+        "host_buffers": host_io,
         "batched": batched,
         "adjoint": adjoint,
         "device": dev,
