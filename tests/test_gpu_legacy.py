@@ -40,6 +40,34 @@ def test_legacy_poisson_sor_log_bits():
         assert same(psi, g[key + "_psi"]), key
 
 
+def test_poisson_host_and_device_entry_points_match_reference():
+    # cnv_poisson_host (host arrays in and out) and cnv_poisson_device (device buffers: here a
+    # second context's w holds f and its u receives psi) give the reference's OpenMP iterate and
+    # logged iteration count, bit for bit, on the same golden cases
+    import ctypes
+    import cnv
+    g = np.load(os.path.join(GOLD, "poisson_cases.npz"))
+    for key in sorted({k.split("_")[0] for k in g.files}):
+        n, beta, omp, tol, k = g[key + "_meta"]
+        if not omp or beta < 0:
+            continue
+        n = int(n)
+        print("case", key, flush=True)
+        f = np.ascontiguousarray(g[key + "_f"], dtype=np.float64)
+        p = cnv.default_params(n, beta=float(beta), tol=float(tol), itmax=200000)
+        with cnv.Cavity(p) as cav, cnv.Cavity(p) as aux:
+            rc, psi, kh, eh = cav.poisson(f)
+            assert rc == cnv.OK and kh == int(k), (key, rc, kh)
+            assert same(psi, g[key + "_psi"]), key
+            aux.set_state(w=f)
+            dw, du = ctypes.c_void_p(), ctypes.c_void_p()
+            assert cnv.lib().cnv_device_state(aux.h, ctypes.byref(dw), None, ctypes.byref(du), None) == cnv.OK
+            kd, ed = ctypes.c_int(), ctypes.c_double()
+            assert cnv.lib().cnv_poisson_device(cav.h, dw, du, ctypes.byref(kd), ctypes.byref(ed)) == cnv.OK
+            assert (kd.value, ed.value) == (kh, eh), key
+            assert same(aux.get_state()["u"], psi), key
+
+
 def test_legacy_euler_continuity_vorticity_bits():
     g = np.load(os.path.join(GOLD, "elementwise.npz"))
     ms = [lb.to_m(g["in_" + k]) for k in ("w", "dwdx", "dwdy", "d2x", "d2y", "u", "v")]
