@@ -122,6 +122,7 @@ def test_driver_help_runs_without_gpu():
     exe = os.path.join(lb.ROOT, "fluid-dynamics1_amd", "cnavier")
     out = subprocess.run([exe, "--help-config"], capture_output=True, text=True, check=True).stdout
     assert "poisson_max_it" in out and "sor_omega" in out
+    assert "output_async" in out and "transport_fp32" in out
 
 
 @pytest.mark.parametrize("order", [3, 8])
