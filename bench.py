@@ -8,9 +8,14 @@ continuity diagnostic).  The same steps are then run in the FP32 + FP64 defect-c
 and compared with the FP64 run.
 
 Prints one JSON line.  value = cells * steps / wall time of the timed steps (HIP events on the
-solver stream), inputs resident in HBM.  roofline: the SOR kernel (the dominant one), algorithmic
-bytes per launch = sweeps_per_launch * 24 B * cells (read psi, read omega, write psi per sweep)
-over its mean launch duration, measured with HIP events on the stream it runs on.
+solver stream), inputs resident in HBM.  roofline: the SOR kernel (the dominant one).  Its
+achieved bandwidth is the bytes one launch must move -- read psi and omega, write psi, once per
+launch: 24 B per cell in FP64, 12 B in FP32 (the S sweeps of a launch run from registers) --
+over its mean launch duration, measured with HIP events on the stream it runs on; frac is that
+over the 8 TB/s HBM peak.  Beside it: the HBM bytes the profiler's counters saw per launch
+(traffic, read from the committed rocprofv3 --pmc summary named in traffic_source), the
+per-sweep rate against the streaming bound of a solver without temporal blocking, and the
+issue-side bound from the committed SQ counter summary.
 cpu_baseline: the oracle (the repository's CPU restatement of the reference step), OpenMP
 red-black, on a bounded sample of the same workload (see "sample"); serial_lexicographic: the
 same on one thread in the reference's default (serial, lexicographic) order; reference_build
@@ -38,6 +43,8 @@ HBM_PEAK = 8.0e12             # MI355X HBM3E, bytes/s
 BYTES_PER_CELL_SWEEP = 24     # read psi, read omega, write psi (FP64)
 BYTES_PER_CELL_SWEEP_FP32 = 12
 C_CELLS = N * N
+CLOCK_HZ = 2.4e9              # MI355X peak engine clock
+SIMDS_PER_CU = 4
 
 
 def workload_params(cnv, precision, args, **kw):
@@ -71,6 +78,72 @@ def read_traffic(name):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def read_sq(name):
+    """Device totals per dispatch (SQ_WAVES, SQ_INSTS_VALU, SQ_WAVE_CYCLES, SQ_WAIT_ANY) from the
+    committed rocprofv3 SQ counter summary, if one exists."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 2 and f[0] in ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY"):
+            try:
+                out.setdefault(f[0], float(f[1]))
+            except ValueError:
+                pass
+    return out if len(out) == 4 else None
+
+
+def roofline(cells, spl, launch_ms, bytes_per_cell, traffic_name, sq_name, waves_per_wg, kernel):
+    """The SOR kernel against HBM: the bytes one launch must move (read psi and the source, write
+    psi, once per launch) over the HIP-event launch time; the counter-measured bytes beside it;
+    the per-sweep rate against a streaming solver's bound (bytes_per_cell per sweep at peak); and
+    the issue bound of the VALU instructions per SIMD from the SQ counters (every VALU
+    instruction issues in one quad-cycle, waves_per_wg / 4 waves per SIMD, one workgroup per CU)."""
+    t = launch_ms / 1e3
+    need = bytes_per_cell * cells
+    traffic = read_traffic(traffic_name)
+    out = {"bound": "hbm", "achieved": need / t / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": need / t / HBM_PEAK, "traffic": traffic,
+           "kernel": kernel, "launch_ms": launch_ms, "bytes_per_launch": need,
+           "bytes_model": "read psi and the source, write psi, once per %d-sweep launch: %d B per cell "
+                          "x %d cells (the sweeps in between run from registers)" % (spl, bytes_per_cell, cells),
+           "traffic_source": "profiles/%s (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each scaled by the same "
+                             "counter on a 256 MiB copy kernel; median working dispatch)" % traffic_name,
+           "counter_GBps": traffic / t / 1e9 if traffic else None,
+           "counter_frac": traffic / t / HBM_PEAK if traffic else None,
+           "sweep_rate": {"cell_sweeps_per_s": spl * cells / t,
+                          "streaming_bound_cell_sweeps_per_s": HBM_PEAK / bytes_per_cell,
+                          "ratio": spl * cells / t / (HBM_PEAK / bytes_per_cell),
+                          "note": "a solver without temporal blocking moves %d B per cell-sweep; this "
+                                  "kernel runs %d sweeps per tile load" % (bytes_per_cell, spl)}}
+    sq = read_sq(sq_name)
+    if sq:
+        per_wave = sq["SQ_INSTS_VALU"] / sq["SQ_WAVES"]
+        issue_s = per_wave * (waves_per_wg / SIMDS_PER_CU) * 4 / CLOCK_HZ
+        out["issue"] = {"valu_per_wave": per_wave, "issue_bound_ms": issue_s * 1e3, "frac": issue_s / t,
+                        "wave_wait_frac": sq["SQ_WAIT_ANY"] / sq["SQ_WAVE_CYCLES"],
+                        "source": "profiles/%s (rocprofv3 --pmc SQ_*), %d waves per workgroup, one "
+                                  "workgroup per CU, one quad-cycle per VALU instruction at %.1f GHz"
+                                  % (sq_name, waves_per_wg, CLOCK_HZ / 1e9)}
+    return out
+
+
+def cpu_whole_step():
+    """The oracle's first time step of the workload from rest, whole and timed (wall vorticity,
+    transport + Euler, the Poisson solve to tol, velocities, continuity), OpenMP red-black on up
+    to 16 host threads: the same arithmetic as the GPU's step, so the same 46352 sweeps."""
+    from oracle import orc
+    threads = min(16, os.cpu_count() or 1)
+    p = orc.make_params(N, Re=RE, dt=DT, order=6, tol=TOL, itmax=50_000_000, beta_=OMEGA,
+                        ordering=orc.REDBLACK, threads=threads)
+    s = orc.State(p)
+    t0 = time.perf_counter()
+    st = s.step()
+    return dict(seconds=time.perf_counter() - t0, sweeps=st.sweeps, threads=threads)
 
 
 def cpu_baseline(sample_seconds=25.0, serial=False):
@@ -247,7 +320,9 @@ def dc_leg(cnv, args, state64, ms64, spl, inner_beta, tag, time_sor=True, transp
     }
     if sor32_ms is not None:
         out["sor_fp32_launch_ms"] = sor32_ms
-        out["sor_fp32_achieved_GBps"] = spl * BYTES_PER_CELL_SWEEP_FP32 * C_CELLS / (sor32_ms / 1e3) / 1e9
+        out["roofline"] = roofline(C_CELLS, spl, sor32_ms, BYTES_PER_CELL_SWEEP_FP32,
+                                   "sor_fp32_refined_n1024_traffic.txt", "sor_fp32_refined_n1024_sq_counters.txt",
+                                   16, "sor_rb_kernel<float,...,S=%d>" % spl)
     print("bench: %s %.3f ms/step, inner sweeps %s" % (tag, ms32 / args.steps, out["inner_sweeps_per_step"]),
           file=sys.stderr, flush=True)
     return out
@@ -317,7 +392,6 @@ def main():
     host_io = None if args.no_host_io else host_buffer_leg(
         cnv, cav, state64, workload_params(cnv, cnv.PREC_FP64, args))
     cav.close()
-    achieved = spl * BYTES_PER_CELL_SWEEP * C / (sor_ms / 1e3)
 
     # ---- the same steps with the opt-in warm start (each solve from the previous psi) ---------
     warm = None
@@ -363,20 +437,31 @@ def main():
     # ---- CPU baseline -----------------------------------------------------------------------
     cpu = None
     if not args.no_cpu:
-        b = cpu_baseline()
+        whole = cpu_whole_step()
+        print("bench: cpu whole step %.1f s, %d sweeps" % (whole["seconds"], whole["sweeps"]),
+              file=sys.stderr, flush=True)
+        cpu = {"value": C / whole["seconds"] / 1e6, "unit": "Mcell-steps/s", "cores": whole["threads"],
+               "kind": "port", "method": "timed",
+               "sample": "oracle (C restatement of the reference step, OpenMP red-black, %d threads): the "
+                         "workload's first time step from rest at 1024^2, whole and timed (%.1f s, %d SOR "
+                         "sweeps; the GPU's timed steps take %d-%d sweeps)"
+                         % (whole["threads"], whole["seconds"], whole["sweeps"], min(sweeps), max(sweeps))}
+        b = cpu_baseline(sample_seconds=8.0)
         kmean = float(np.mean(sweeps))
         t_step = b["t_transport"] + kmean * b["t_sweep"]
-        cpu = {"value": C / t_step / 1e6, "unit": "Mcell-steps/s", "cores": b["threads"], "kind": "port",
-               "sample": "oracle (C restatement, OpenMP red-black, %d threads) at 1024^2: step-0 transport "
-                         "(%.3f s) + %d SOR sweeps (%.2f ms/sweep), scaled by the GPU run's mean %.0f sweeps/step"
-                         % (b["threads"], b["t_transport"], b["sweeps"], b["t_sweep"] * 1e3, kmean)}
+        cpu["modelled_timed_window"] = {
+            "value": C / t_step / 1e6, "unit": "Mcell-steps/s", "cores": b["threads"], "method": "model",
+            "sample": "model, not a timed step: oracle (OpenMP red-black, %d threads) step-0 transport (%.3f s) "
+                      "+ %d timed SOR sweeps (%.2f ms/sweep), scaled by the GPU's mean %.0f sweeps per timed step"
+                      % (b["threads"], b["t_transport"], b["sweeps"], b["t_sweep"] * 1e3, kmean)}
         bs = cpu_baseline(sample_seconds=8.0, serial=True)
         t_step_s = bs["t_transport"] + kmean * bs["t_sweep"]
         cpu["serial_lexicographic"] = {
-            "value": C / t_step_s / 1e6, "unit": "Mcell-steps/s", "cores": 1,
-            "sample": "oracle, one thread, lexicographic SOR (the reference's default build) at 1024^2: "
-                      "step-0 transport (%.3f s) + %d SOR sweeps (%.2f ms/sweep), scaled by the GPU run's "
-                      "mean %.0f sweeps/step" % (bs["t_transport"], bs["sweeps"], bs["t_sweep"] * 1e3, kmean)}
+            "value": C / t_step_s / 1e6, "unit": "Mcell-steps/s", "cores": 1, "method": "model",
+            "sample": "model, not a timed step: oracle, one thread, lexicographic SOR (the reference's default "
+                      "build) at 1024^2: step-0 transport (%.3f s) + %d timed SOR sweeps (%.2f ms/sweep), scaled "
+                      "by the GPU's mean %.0f sweeps per timed step"
+                      % (bs["t_transport"], bs["sweeps"], bs["t_sweep"] * 1e3, kmean)}
         from oracle import ref_sor_sample
         if ref_sor_sample.available():
             # the reference's own poisson_SOR_log, compiled from its sources (oracle/_ref); its
@@ -385,7 +470,8 @@ def main():
             t_step_r = b["t_transport"] + kmean * t_ref
             cpu["reference_build"] = {
                 "value": C / t_step_r / 1e6, "unit": "Mcell-steps/s", "cores": b["threads"], "kind": "reference",
-                "sample": "the reference's own poisson_SOR_log (src/poisson.c, built by oracle/Makefile with its "
+                "method": "model",
+                "sample": "model, not a timed step: the reference's own poisson_SOR_log (src/poisson.c, built by oracle/Makefile with its "
                           "Makefile's flags -g -fopenmp and -DOPENMP_ENABLED: red-black, %d threads) at 1024^2: "
                           "%d sweeps (%.2f ms/sweep) on the step-0 source, plus the oracle's step-0 transport "
                           "(%.3f s; the reference's dense Kronecker transport cannot be formed at 1024^2), "
@@ -394,7 +480,6 @@ def main():
         else:
             print("bench: oracle/_ref not built; reference_build baseline skipped", file=sys.stderr, flush=True)
 
-    traffic = read_traffic("sor_fp64_n1024_traffic.txt")
     line = {
         "metric": "Mcell-steps/s & %HBM peak, 1024^2 Re1000 cavity; FP32-refined vs FP64 plain",
         "value": value,
@@ -413,9 +498,8 @@ def main():
                    "step": "eager launches" if args.eager else "captured HIP graphs",
                    "sweeps_per_launch": spl, "sweeps_per_step": sweeps,
                    "algorithmic_GBps_step_loop": alg_bytes / T / 1e9},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "sor_rb_kernel<double,...,S=%d>" % spl, "launch_ms": sor_ms},
+        "roofline": roofline(C, spl, sor_ms, BYTES_PER_CELL_SWEEP, "sor_fp64_n1024_traffic.txt",
+                             "sor_fp64_n1024_sq_counters.txt", 16, "sor_rb_kernel<double,...,S=%d>" % spl),
         "cpu_baseline": cpu,
         "fp64_warm_start": warm,
         "fp32_refined": dc,

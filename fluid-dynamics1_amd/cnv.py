@@ -4,14 +4,16 @@ This module is the Python face of the C ABI for tests and bench.py; it allocates
 GPU itself and has no fallback: if libcnv.so is missing or no HIP device is present, calls raise.
 """
 import ctypes as C
+import json
 import os
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcnv.so")
+WORKLOADS = os.path.join(HERE, "workloads")
 
-OK, ERR_ARG, ERR_HIP, NOT_CONVERGED, ERR_NOMEM, ERR_NO_DEVICE = 0, 1, 2, 3, 4, 5
+OK, ERR_ARG, ERR_HIP, NOT_CONVERGED, ERR_NOMEM, ERR_NO_DEVICE, DIVERGED = 0, 1, 2, 3, 4, 5, 6
 PREC_FP64, PREC_FP32_DC = 0, 1
 
 
@@ -27,6 +29,7 @@ class Params(C.Structure):
         ("sweeps_per_launch", C.c_int), ("tile_waves", C.c_int), ("wave_rows", C.c_int),
         ("launches_per_poll", C.c_int), ("diagnostics", C.c_int), ("use_graph", C.c_int),
         ("inner_beta", C.c_double), ("warm_start", C.c_int), ("transport_fp32", C.c_int),
+        ("blowup_speed", C.c_double),
     ]
 
 
@@ -89,6 +92,8 @@ def lib():
         L.cnv_batch_step.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepStats)]
         L.cnv_batch_step_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(StepStats), C.POINTER(C.c_double)]
         L.cnv_batch_get_state.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
+        L.cnv_batch_set_state.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 4
+        L.cnv_batch_status.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.cnv_batch_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.cnv_adj_create.argtypes = [C.POINTER(Params), C.c_int, C.c_double, C.c_double, C.POINTER(C.c_void_p)]
@@ -285,6 +290,21 @@ class Batch:
         self._check(lib().cnv_batch_get_state(self.h, i, *[a.ctypes.data for a in out]))
         return dict(zip(("w", "psi", "u", "v"), out))
 
+    def set_state(self, i, w=None, psi=None, u=None, v=None):
+        n = self.ns[i]
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (w, psi, u, v)]
+        for a in arrs:
+            assert a is None or a.shape == (n, n)
+        self._check(lib().cnv_batch_set_state(self.h, i, *[None if a is None else a.ctypes.data for a in arrs]))
+
+    def status(self):
+        """Per-cavity CNV_OK (live) or CNV_DIVERGED."""
+        st = (C.c_int * len(self.ns))()
+        n = lib().cnv_batch_status(self.h, st)
+        if n < 0:
+            self._check(-n)
+        return list(st)
+
     def autotune(self, launches=20):
         s, w, r, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         self._check(lib().cnv_batch_autotune(self.h, launches, C.byref(s), C.byref(w), C.byref(r),
@@ -345,27 +365,44 @@ class Adjoint:
         return f.value, b.value
 
 
-def gaussian_vorticity(n, count=32, seed=1):
-    """Initial vorticity for the adjoint workload: a sum of `count` Gaussians drawn with numpy's
-    default_rng(seed) -- centres uniform on the unit square, widths uniform on [0.02, 0.1],
-    amplitudes N(0, 1) -- sampled at the nodes x = j/n, y = i/n (src/main.c:138-139)."""
+def gaussian_parameters(count=32, seed=1):
+    """The adjoint workload's 32 Gaussians: (cx, cy, width, amplitude) arrays.  (32, seed 1) comes
+    from the committed table workloads/gaussians32_seed1.json; any other (count, seed) is drawn
+    with numpy's default_rng(seed) -- centres uniform on the unit square, widths uniform on
+    [0.02, 0.1], amplitudes N(0, 1) -- the draw that wrote the table (workloads/make_tables.py)."""
+    if (count, seed) == (32, 1):
+        with open(os.path.join(WORKLOADS, "gaussians32_seed1.json")) as f:
+            t = json.load(f)
+        return tuple(np.array(t[k], dtype=np.float64) for k in ("cx", "cy", "width", "amplitude"))
     rng = np.random.default_rng(seed)
     cx, cy = rng.uniform(0.0, 1.0, count), rng.uniform(0.0, 1.0, count)
     width = rng.uniform(0.02, 0.1, count)
     amp = rng.standard_normal(count)
+    return cx, cy, width, amp
+
+
+def gaussian_vorticity(n, count=32, seed=1):
+    """Initial vorticity for the adjoint workload: the sum of the Gaussians of
+    gaussian_parameters(count, seed), sampled at the nodes x = j/n, y = i/n (src/main.c:138-139)."""
+    cx, cy, width, amp = gaussian_parameters(count, seed)
     x = np.arange(n) / n
     X, Y = np.meshgrid(x, x)  # X[i, j] = j/n, Y[i, j] = i/n
     w = np.zeros((n, n))
-    for k in range(count):
+    for k in range(len(amp)):
         w += amp[k] * np.exp(-((X - cx[k]) ** 2 + (Y - cy[k]) ** 2) / (2.0 * width[k] ** 2))
     return w
 
 
 def batch_table(count=64, seed=0):
-    """The batched workload: `count` lid-driven cavities drawn with numpy's default_rng(seed):
-    side N uniform over {64, 96, 128, 192, 256, 384, 512}, Re log-uniform on [100, 5000], lid
-    velocity uniform on [0.5, 1.5]; dt = 0.2*min(h*h*Re/4, h) with h = 1/N; SOR relaxation from
-    the reference's rule for the grid side (src/main.c:134)."""
+    """The batched workload: `count` lid-driven cavities.  (64, seed 0) comes from the committed
+    table workloads/batch64_seed0.json; any other (count, seed) is drawn with numpy's
+    default_rng(seed) -- side N uniform over {64, 96, 128, 192, 256, 384, 512}, Re log-uniform on
+    [100, 5000], lid velocity uniform on [0.5, 1.5] -- the draw that wrote the table
+    (workloads/make_tables.py); dt = 0.2*min(h*h*Re/4, h) with h = 1/N, SOR relaxation from the
+    reference's rule for the grid side (src/main.c:134)."""
+    if (count, seed) == (64, 0):
+        with open(os.path.join(WORKLOADS, "batch64_seed0.json")) as f:
+            return json.load(f)["rows"]
     rng = np.random.default_rng(seed)
     sides = np.array([64, 96, 128, 192, 256, 384, 512])
     N = rng.choice(sides, size=count)

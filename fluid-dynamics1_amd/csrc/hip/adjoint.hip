@@ -210,6 +210,7 @@ struct cnv_adjoint {
     double* d_seg = nullptr;   // [K][3][cells]
     double *lw = nullptr, *lu = nullptr, *lv = nullptr, *lwt = nullptr, *lub = nullptr, *lvb = nullptr;
     double *wt = nullptr, *src = nullptr, *dU = nullptr;
+    double *uT = nullptr, *vT = nullptr;  // the last forward run's final velocities (the seed)
     int k_fwd = 0, k_adj = 0;  // queue-ahead predictions of the two kinds of solve
     long long sweeps_fwd = 0, sweeps_adj = 0;
     char err[256] = "";
@@ -255,6 +256,8 @@ static int solve_rel(cnv_adjoint* a, const double* src, double rtol, int* kpred,
     SorCtl fin{};
     c->k_prev = *kpred;
     ACTX(a, run_sor<double>(c, sa, &fin));
+    if (fin.status == SOR_DIVERGED)
+        return afail(a, CNV_DIVERGED, "adjoint-problem Poisson solve diverged (non-finite update) after %d sweeps", fin.k);
     if (fin.status == SOR_FAILED)
         return afail(a, CNV_NOT_CONVERGED, "adjoint-problem Poisson solve did not converge in %d sweeps", fin.k);
     *kpred = fin.k;
@@ -377,7 +380,7 @@ void cnv_adj_destroy(cnv_adjoint* a) {
         cnv_destroy(a->c);
     }
     void* dev[] = {a->d_tstart, a->d_ttab, a->d_ckpt, a->d_seg, a->lw, a->lu, a->lv, a->lwt, a->lub, a->lvb,
-                   a->wt, a->src, a->dU};
+                   a->wt, a->src, a->dU, a->uT, a->vT};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     delete a;
@@ -403,7 +406,7 @@ int cnv_adj_create(const cnv_params* p, int checkpoint_every, double rtol_forwar
         a->cells = a->c->cells;
         const size_t bytes = sizeof(double) * a->cells;
         AHIP(a, hipMalloc(&a->d_seg, bytes * 3 * a->K));
-        double** bufs[] = {&a->lw, &a->lu, &a->lv, &a->lwt, &a->lub, &a->lvb, &a->wt, &a->src};
+        double** bufs[] = {&a->lw, &a->lu, &a->lv, &a->lwt, &a->lub, &a->lvb, &a->wt, &a->src, &a->uT, &a->vT};
         for (double** b : bufs) AHIP(a, hipMalloc(b, bytes));
         AHIP(a, hipMalloc(&a->dU, sizeof(double)));
         return build_transposed(a);
@@ -456,6 +459,9 @@ int cnv_adj_forward(cnv_adjoint* a, const double* w0, double U, int nsteps, doub
         int rc = fwd_step(a);
         if (rc) return rc;
     }
+    // the backward pass seeds from these: its recompute moves the context's own state
+    AHIP(a, hipMemcpyAsync(a->uT, c->u, bytes, hipMemcpyDeviceToDevice, c->stream));
+    AHIP(a, hipMemcpyAsync(a->vT, c->v, bytes, hipMemcpyDeviceToDevice, c->stream));
     // E = 1/2 dx dy sum (u^2 + v^2), summed on the host in row-major order
     AHIP(a, hipMemcpyAsync(u0.data(), c->u, bytes, hipMemcpyDeviceToHost, c->stream));
     AHIP(a, hipMemcpyAsync(v0.data(), c->v, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -472,8 +478,10 @@ int cnv_adj_backward(cnv_adjoint* a, double* dE_dw0, double* dE_dU) {
     AHIP(a, hipSetDevice(c->p.device));
     const int n = c->n, ld = c->ld;
     const dim3 g = cell_grid(n), blk(kBlock);
-    // seed from the final state (the context holds it after the forward run)
-    hipLaunchKernelGGL(adj_seed_kernel, g, blk, 0, c->stream, c->u, c->v, a->lw, a->lu, a->lv, n, ld,
+    // seed from the last forward run's final state, kept apart from the context's state, which
+    // each backward pass's recompute moves (to step min(T, K) by its end): a second backward
+    // call after one forward run returns the same gradient
+    hipLaunchKernelGGL(adj_seed_kernel, g, blk, 0, c->stream, a->uT, a->vT, a->lw, a->lu, a->lv, n, ld,
                        c->p.dx * c->p.dy);
     ALAUNCH(a);
     hipLaunchKernelGGL(adj_zero_kernel, dim3(1), dim3(kBlock), 0, c->stream, a->dU, 1LL);

@@ -15,6 +15,10 @@
 // init for every cavity still iterating, SOR launches until every inner solve has ended, then
 // the FP64 correction.  Cavities that converged drop out of the later outer iterations.
 //
+// A cavity that diverges -- a non-finite L1 update ends its solve (SOR_DIVERGED), or its largest
+// |u| or |v| after a step exceeds its blowup_speed -- is taken out of the workgroup maps: later
+// steps launch nothing for it and the other cavities go on (cnv_batch_status reports it).
+//
 // The SOR launch shape (sweeps per launch, waves, rows per wave) is one per batch; the autotuner
 // (cnv_batch_autotune) times every compiled shape on the batch itself and keeps the fastest.
 #pragma clang fp contract(off)
@@ -129,6 +133,7 @@ __global__ void __launch_bounds__(kBlock) batch_continuity_kernel(const StepItem
     if (m.inst < 0) return;  // uniform per workgroup: the block-wide fold below is safe
     const StepItem& I = it[m.inst];
     double mx = -DBL_MAX, mn = DBL_MAX;
+    unsigned long long ks = dkey(0.0);
     const int n = I.n;
     const long long count = (long long)n * n;
     for (long long q = (long long)m.lb * kBlock + threadIdx.x; q < count; q += (long long)m.nb * kBlock) {
@@ -136,8 +141,11 @@ __global__ void __launch_bounds__(kBlock) batch_continuity_kernel(const StepItem
         const double c = continuity_at(I.u, I.v, I.ld, I.st, i, j);
         if (c > mx) mx = c;
         if (c < mn) mn = c;
+        const size_t p = (size_t)i * I.ld + j;
+        const unsigned long long kq = speed_key(I.u[p], I.v[p]);
+        ks = kq > ks ? kq : ks;
     }
-    diag_fold_block(mx, mn, I.diag);
+    diag_fold_block(mx, mn, ks, I.diag);
 }
 
 __global__ void __launch_bounds__(kBlock) batch_dc_residual_kernel(const StepItem* __restrict__ it,
@@ -229,6 +237,9 @@ thread_local char g_berr[256] = "";  // last error of this thread
 struct cnv_batch {
     int count = 0;
     int dc = 0, gs = 0;
+    std::vector<int> alive;        // 1: the cavity takes part in the steps; 0: it diverged
+    std::vector<double> blowup;    // per cavity: blowup_speed (0 = no check)
+    bool maps_dirty = false;       // alive changed: rebuild the workgroup maps before the next step
     bool transport_fp32 = false;  // refined form: FP32 transport stencils (default FP64)
     int device = 0;
     hipStream_t stream = nullptr;
@@ -305,6 +316,7 @@ static int upload(cnv_batch* b, V** dst, const std::vector<V>& src) {
 static int build_maps(cnv_batch* b) {
     std::vector<BlockRef> edge, cell, lin;
     for (int i = 0; i < b->count; ++i) {
+        if (!b->alive[i]) continue;
         const int n = b->ctx[i]->n;
         const int ne = (4 * n + kBlock - 1) / kBlock;
         for (int l = 0; l < ne; ++l) edge.push_back({i, l, ne});
@@ -331,7 +343,8 @@ static int build_sor(cnv_batch* b, TileShape sh) {
     std::vector<int> ti(b->count), tj(b->count);
     for (int i = 0; i < b->count; ++i) {
         const int nt = tiles_of(b->ctx[i]->n, sh, &ti[i], &tj[i]);
-        for (int t = 0; t < nt; ++t) list.push_back({i, t});
+        if (b->alive[i])
+            for (int t = 0; t < nt; ++t) list.push_back({i, t});
     }
     while (list.size() % 8) list.push_back({-1, 0});
     const int per = (int)list.size() / 8;
@@ -428,7 +441,7 @@ static int run_sor_batch(cnv_batch* b, bool fp32, std::vector<SorCtl>& snaps) {
             BHIP(b, hipEventSynchronize(b->ev[s0]));
             const SorCtl* h = b->h_ctl + s0 * per;
             bool all = true;
-            for (int i = 0; i < b->count && all; ++i) all = h[2 * i + f0].status != SOR_RUNNING;
+            for (int i = 0; i < b->count && all; ++i) all = !b->alive[i] || h[2 * i + f0].status != SOR_RUNNING;
             if (all) {
                 snaps.resize(b->count);
                 for (int i = 0; i < b->count; ++i) snaps[i] = h[2 * i + f0];
@@ -443,6 +456,21 @@ static int run_sor_batch(cnv_batch* b, bool fp32, std::vector<SorCtl>& snaps) {
 static int batch_one_step(cnv_batch* b, cnv_step_stats* stats) {
     const int cnt = b->count;
     const int win = b->wcur;
+    if (b->maps_dirty) {  // a cavity diverged (or came back): maps of the live cavities only
+        int rc = build_maps(b);
+        if (!rc) rc = build_sor(b, b->shape);
+        if (rc) return rc;
+        b->maps_dirty = false;
+    }
+    int nalive = 0;
+    for (int i = 0; i < cnt; ++i) nalive += b->alive[i];
+    if (nalive == 0) {  // nothing left to step
+        for (int i = 0; i < cnt && stats; ++i) {
+            stats[i] = cnv_step_stats{};
+            stats[i].status = CNV_DIVERGED;
+        }
+        return CNV_OK;
+    }
     hipLaunchKernelGGL(batch_wall_vorticity_kernel, dim3(b->nb_edge), dim3(kBlock), 0, b->stream, b->d_items,
                        b->d_map_edge, win);
     BLAUNCH(b);
@@ -454,13 +482,14 @@ static int batch_one_step(cnv_batch* b, cnv_step_stats* stats) {
                            b->d_map_cell, win);
     BLAUNCH(b);
     b->wcur ^= 1;
-    for (cnv_ctx* c : b->ctx) c->wcur = b->wcur;
+    for (int i = 0; i < cnt; ++i)  // a diverged cavity keeps the buffers it was left in
+        if (b->alive[i]) b->ctx[i]->wcur = b->wcur;
     hipLaunchKernelGGL(batch_poisson_init_kernel, dim3(b->nb_lin), dim3(kBlock), 0, b->stream, b->d_items,
                        b->d_map_lin);
     BLAUNCH(b);
 
     std::vector<SorCtl> fin(cnt), snaps;
-    std::vector<int> outer(cnt, 0), total(cnt, 0), active(cnt, 1);
+    std::vector<int> outer(cnt, 0), total(cnt, 0), active(b->alive);
     if (!b->dc) {
         int rc = run_sor_batch(b, false, snaps);
         if (rc) return rc;
@@ -505,14 +534,15 @@ static int batch_one_step(cnv_batch* b, cnv_step_stats* stats) {
                 outer[i] = o + 1;
                 fin[i] = snaps[i];
                 fin[i].k = total[i];
-                if (snaps[i].status == SOR_CONVERGED || snaps[i].status == SOR_FAILED) active[i] = 0;
+                if (dc_final(snaps[i].status)) active[i] = 0;
             }
         }
         for (int i = 0; i < cnt; ++i)
             if (active[i]) fin[i].status = SOR_FAILED;  // ran out of outer iterations
     }
     int kmax = 0;
-    for (int i = 0; i < cnt; ++i) kmax = std::max(kmax, b->dc ? 0 : fin[i].k);
+    for (int i = 0; i < cnt; ++i)
+        if (b->alive[i]) kmax = std::max(kmax, b->dc ? 0 : fin[i].k);
     if (!b->dc) b->k_prev = kmax;
 
     if (b->dc)
@@ -529,15 +559,27 @@ static int batch_one_step(cnv_batch* b, cnv_step_stats* stats) {
     int status = CNV_OK;
     for (int i = 0; i < cnt; ++i) {
         cnv_step_stats s{};
-        s.status = fin[i].status == SOR_CONVERGED ? CNV_OK : CNV_NOT_CONVERGED;
+        if (!b->alive[i]) {  // diverged in an earlier step: not stepped
+            s.status = CNV_DIVERGED;
+            if (stats) stats[i] = s;
+            continue;
+        }
+        s.status = solve_status(fin[i].status);
         s.sweeps = fin[i].k;
         s.k = fin[i].k - 1;
         s.e = fin[i].e_last;
         s.outer = outer[i];
         s.cont_max = dkey_inv(b->h_diag[i].kmax);
         s.cont_min = dkey_inv(b->h_diag[i].kmin);
+        const double speed = dkey_inv(b->h_diag[i].kspd);
+        if (s.status == CNV_OK && b->blowup[i] > 0 && !(speed <= b->blowup[i])) s.status = CNV_DIVERGED;
         if (stats) stats[i] = s;
-        if (s.status != CNV_OK) status = CNV_NOT_CONVERGED;
+        if (s.status == CNV_DIVERGED) {
+            b->alive[i] = 0;
+            b->maps_dirty = true;
+        } else if (s.status != CNV_OK) {
+            status = CNV_NOT_CONVERGED;
+        }
         b->ctx[i]->fin_slot = b->fin_slot;
     }
     if (status != CNV_OK)
@@ -589,6 +631,9 @@ int cnv_batch_create(const cnv_params* items, int count, cnv_batch** out) {
             return bfail(nullptr, CNV_ERR_ARG, "batch item %d: warm_start is for one-cavity contexts only", i);
     cnv_batch* b = new cnv_batch();
     b->count = count;
+    b->alive.assign(count, 1);
+    b->blowup.resize(count);
+    for (int i = 0; i < count; ++i) b->blowup[i] = items[i].blowup_speed;
     b->dc = p0.precision == CNV_PREC_FP32_DC;
     b->transport_fp32 = p0.transport_fp32 != 0;
     b->gs = p0.poisson_type == 1;
@@ -700,15 +745,41 @@ int cnv_batch_get_state(cnv_batch* b, int i, double* w, double* psi, double* u, 
     BHIP(b, hipStreamSynchronize(b->stream));
     cnv_ctx* c = b->ctx[i];
     // the cavity's final iterate: psi[0] for defect correction, else the buffer its last slot names
+    // (c->fin_slot and c->wcur: as the cavity's last step left them, also after it diverged)
     SorCtl sl{};
-    BHIP(b, hipMemcpy(&sl, b->d_ctl + 2 * i + b->fin_slot, sizeof(SorCtl), hipMemcpyDeviceToHost));
+    BHIP(b, hipMemcpy(&sl, b->d_ctl + 2 * i + c->fin_slot, sizeof(SorCtl), hipMemcpyDeviceToHost));
     const double* psrc = b->dc ? c->psi[0] : c->psi[sl.parity & 1];
     const size_t bytes = sizeof(double) * c->cells;
     if (psi) BHIP(b, hipMemcpy(psi, psrc, bytes, hipMemcpyDeviceToHost));
-    if (w) BHIP(b, hipMemcpy(w, c->w[b->wcur], bytes, hipMemcpyDeviceToHost));
+    if (w) BHIP(b, hipMemcpy(w, c->w[c->wcur], bytes, hipMemcpyDeviceToHost));
     if (u) BHIP(b, hipMemcpy(u, c->u, bytes, hipMemcpyDeviceToHost));
     if (v) BHIP(b, hipMemcpy(v, c->v, bytes, hipMemcpyDeviceToHost));
     return CNV_OK;
+}
+
+int cnv_batch_set_state(cnv_batch* b, int i, const double* w, const double* psi, const double* u, const double* v) {
+    if (!b || i < 0 || i >= b->count) return bfail(b, CNV_ERR_ARG, "batch index %d out of range", i);
+    BHIP(b, hipSetDevice(b->device));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    cnv_ctx* c = b->ctx[i];
+    c->wcur = b->wcur;  // the batch's current w buffer
+    const int rc = cnv_set_state(c, w, psi, u, v);
+    if (rc) return bfail(b, rc, "batch item %d: %s", i, cnv_last_error(c));
+    if (!b->alive[i]) {
+        b->alive[i] = 1;
+        b->maps_dirty = true;
+    }
+    return CNV_OK;
+}
+
+int cnv_batch_status(const cnv_batch* b, int* status) {
+    if (!b) return -bfail(nullptr, CNV_ERR_ARG, "null batch");
+    int n = 0;
+    for (int i = 0; i < b->count; ++i) {
+        if (status) status[i] = b->alive[i] ? CNV_OK : CNV_DIVERGED;
+        n += b->alive[i];
+    }
+    return n;
 }
 
 int cnv_batch_autotune(cnv_batch* b, int launches, int* sweeps_per_launch, int* tile_waves, int* wave_rows,
@@ -716,21 +787,17 @@ int cnv_batch_autotune(cnv_batch* b, int launches, int* sweeps_per_launch, int* 
     if (!b || launches < 4) return bfail(b, CNV_ERR_ARG, "autotune needs at least 4 launches");
     BHIP(b, hipSetDevice(b->device));
     BHIP(b, hipStreamSynchronize(b->stream));
-    // synthetic sources in the planes the solve reads: FP64 w[wcur] (negated by the solve) or
-    // the FP32 residual r32
-    std::vector<void*> dst(b->count);
-    for (int i = 0; i < b->count; ++i)
-        dst[i] = b->dc ? (void*)b->ctx[i]->r32 : (void*)b->ctx[i]->w[b->wcur ^ 1];
-    void** d_dst = nullptr;
-    BHIP(b, hipMalloc(&d_dst, sizeof(void*) * b->count));
-    BHIP(b, hipMemcpy(d_dst, dst.data(), sizeof(void*) * b->count, hipMemcpyHostToDevice));
-    if (b->dc)
-        hipLaunchKernelGGL(batch_fill_source_kernel<float>, dim3(b->nb_lin), dim3(kBlock), 0, b->stream, b->d_items,
-                           b->d_map_lin, (float**)d_dst);
-    else
-        hipLaunchKernelGGL(batch_fill_source_kernel<double>, dim3(b->nb_lin), dim3(kBlock), 0, b->stream, b->d_items,
-                           b->d_map_lin, (double**)d_dst);
-    BLAUNCH(b);
+    if (b->maps_dirty) {
+        int rc = build_maps(b);
+        if (!rc) rc = build_sor(b, b->shape);
+        if (rc) return rc;
+        b->maps_dirty = false;
+    }
+    if (cnv_batch_status(b, nullptr) == 0) return bfail(b, CNV_ERR_ARG, "autotune: every cavity has diverged");
+    // the slots are overwritten below; a diverged cavity's (which say where its last iterate
+    // is) are put back afterwards
+    std::vector<SorCtl> ctl_keep((size_t)2 * b->count);
+    BHIP(b, hipMemcpy(ctl_keep.data(), b->d_ctl, sizeof(SorCtl) * ctl_keep.size(), hipMemcpyDeviceToHost));
     const TileShape cands[] = {
 #define CNV_SOR_LIST(s_, nw_, rb_) TileShape{s_, nw_, rb_},
         CNV_SOR_SHAPES(CNV_SOR_LIST)
@@ -738,44 +805,74 @@ int cnv_batch_autotune(cnv_batch* b, int launches, int* sweeps_per_launch, int* 
     };
     const TileShape keep = b->shape;
     const int wkeep = b->wcur;
-    b->wcur ^= 1;  // the solve reads w[wcur]: point it at the synthetic field
     TileShape best = keep;
     double best_ms = 1e30;
-    // every launch sweeps S times: slots that no sweep can stop
-    for (const TileShape& sh : cands) {
-        int rc = build_sor(b, sh);
-        if (rc) return rc;
-        hipLaunchKernelGGL(batch_tune_slots_kernel, dim3((2 * b->count + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                           b->stream, b->d_ctl, 2 * b->count);
+    void** d_dst = nullptr;
+    auto tune = [&]() -> int {
+        // synthetic sources in the planes the solve reads: FP64 w[wcur] (negated by the solve) or
+        // the FP32 residual r32
+        std::vector<void*> dst(b->count);
+        for (int i = 0; i < b->count; ++i)
+            dst[i] = b->dc ? (void*)b->ctx[i]->r32 : (void*)b->ctx[i]->w[b->wcur ^ 1];
+        BHIP(b, hipMalloc(&d_dst, sizeof(void*) * b->count));
+        BHIP(b, hipMemcpy(d_dst, dst.data(), sizeof(void*) * b->count, hipMemcpyHostToDevice));
+        if (b->dc)
+            hipLaunchKernelGGL(batch_fill_source_kernel<float>, dim3(b->nb_lin), dim3(kBlock), 0, b->stream,
+                               b->d_items, b->d_map_lin, (float**)d_dst);
+        else
+            hipLaunchKernelGGL(batch_fill_source_kernel<double>, dim3(b->nb_lin), dim3(kBlock), 0, b->stream,
+                               b->d_items, b->d_map_lin, (double**)d_dst);
         BLAUNCH(b);
-        for (int l = 0; l < 2; ++l) {
-            rc = sor_launch(b, l, b->dc);
+        b->wcur ^= 1;  // the solve reads w[wcur]: point it at the synthetic field
+        // every launch sweeps S times: slots that no sweep can stop
+        for (const TileShape& sh : cands) {
+            int rc = build_sor(b, sh);
             if (rc) return rc;
+            hipLaunchKernelGGL(batch_tune_slots_kernel, dim3((2 * b->count + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                               b->stream, b->d_ctl, 2 * b->count);
+            BLAUNCH(b);
+            for (int l = 0; l < 2; ++l) {
+                rc = sor_launch(b, l, b->dc);
+                if (rc) return rc;
+            }
+            BHIP(b, hipEventRecord(b->t0, b->stream));
+            for (int l = 2; l < launches; ++l) {
+                rc = sor_launch(b, l, b->dc);
+                if (rc) return rc;
+            }
+            BHIP(b, hipEventRecord(b->t1, b->stream));
+            BHIP(b, hipEventSynchronize(b->t1));
+            float ms = 0.f;
+            BHIP(b, hipEventElapsedTime(&ms, b->t0, b->t1));
+            const double per_sweep = ms / ((launches - 2) * (double)sh.s);
+            if (per_sweep < best_ms) {
+                best_ms = per_sweep;
+                best = sh;
+            }
         }
-        BHIP(b, hipEventRecord(b->t0, b->stream));
-        for (int l = 2; l < launches; ++l) {
-            rc = sor_launch(b, l, b->dc);
-            if (rc) return rc;
-        }
-        BHIP(b, hipEventRecord(b->t1, b->stream));
-        BHIP(b, hipEventSynchronize(b->t1));
-        float ms = 0.f;
-        BHIP(b, hipEventElapsedTime(&ms, b->t0, b->t1));
-        const double per_sweep = ms / ((launches - 2) * (double)sh.s);
-        if (per_sweep < best_ms) {
-            best_ms = per_sweep;
-            best = sh;
-        }
-    }
+        return CNV_OK;
+    };
+    const int rc_tune = tune();
+    // on every exit: the batch's own w buffer, the chosen (or, after a failure, the kept) shape,
+    // zeroed iterates of the live cavities, the diverged cavities' slots, no scratch left behind
+    char msg[256];
+    snprintf(msg, sizeof(msg), "%s", b->err);
+    (void)hipStreamSynchronize(b->stream);
     b->wcur = wkeep;
-    (void)hipFree(d_dst);
-    // restore: the chosen shape, zeroed iterates, fresh sources for the next step
-    int rc = build_sor(b, best);
-    if (rc) return rc;
-    hipLaunchKernelGGL(batch_zero_kernel, dim3(b->nb_lin), dim3(kBlock), 0, b->stream, b->d_items, b->d_map_lin);
-    BLAUNCH(b);
-    BHIP(b, hipStreamSynchronize(b->stream));
+    if (d_dst) (void)hipFree(d_dst);
+    int rc = build_sor(b, rc_tune ? keep : best);
+    if (!rc) {
+        hipLaunchKernelGGL(batch_zero_kernel, dim3(b->nb_lin), dim3(kBlock), 0, b->stream, b->d_items, b->d_map_lin);
+        BLAUNCH(b);
+        for (int i = 0; i < b->count; ++i)
+            if (!b->alive[i])
+                BHIP(b, hipMemcpyAsync(b->d_ctl + 2 * i, ctl_keep.data() + 2 * i, 2 * sizeof(SorCtl),
+                                       hipMemcpyHostToDevice, b->stream));
+        BHIP(b, hipStreamSynchronize(b->stream));
+    }
     b->k_prev = 0;
+    if (rc_tune) return bfail(b, rc_tune, "%s", msg);
+    if (rc) return rc;
     if (sweeps_per_launch) *sweeps_per_launch = best.s;
     if (tile_waves) *tile_waves = best.nw;
     if (wave_rows) *wave_rows = best.rb;

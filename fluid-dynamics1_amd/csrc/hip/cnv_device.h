@@ -16,7 +16,8 @@ enum SorStatus : int {
     SOR_RUNNING = 0,
     SOR_CONVERGED = 1,  // a sweep's L1 update fell below tol
     SOR_FAILED = 2,     // itmax sweeps without convergence
-    SOR_FOLD = 4        // defect correction: the FP32 iterate reached its resolution (rtol)
+    SOR_FOLD = 4,       // defect correction: the FP32 iterate reached its resolution (rtol)
+    SOR_DIVERGED = 8    // a sweep's L1 update (or the iterate's norm) was NaN or infinite
 };
 
 struct SorCtl {
@@ -32,10 +33,13 @@ struct SorCtl {
     double n_last;      // L1 norm of the iterate after the last accepted sweep (FP32 planes)
 };
 
-// Continuity diagnostic: max/min as order-preserving unsigned keys (exact, order independent).
+// Continuity diagnostic: max/min as order-preserving unsigned keys (exact, order independent),
+// and the largest |u| or |v| of the stored velocities (a NaN orders above +infinity), which the
+// blow-up check reads.
 struct Diag {
     unsigned long long kmax;
     unsigned long long kmin;
+    unsigned long long kspd;
 };
 
 __host__ __device__ inline unsigned long long dkey(double x) {

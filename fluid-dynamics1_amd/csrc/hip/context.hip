@@ -81,6 +81,7 @@ const char* cnv_status_string(int s) {
         case CNV_NOT_CONVERGED: return "Poisson solve did not converge within poisson_max_it";
         case CNV_ERR_NOMEM: return "out of memory";
         case CNV_ERR_NO_DEVICE: return "no HIP device";
+        case CNV_DIVERGED: return "diverged (non-finite Poisson update, or blow-up speed exceeded)";
         default: return "unknown status";
     }
 }
@@ -102,7 +103,7 @@ int cnv_device_name(int device, char* buf, int len) {
 // sweep, FP64: 128^2 and 256^2 best with (5, 4, 6), 512^2 with (5, 8, 4), 768^2 with (6, 16, 4)
 // (2.21 us against 2.42 us with (5, 16, 4)), 1024^2 with (5, 16, 4) ((6, 16, 4) needs 260 tiles
 // there).  Grids beyond take (5, 16, 4), the thickest tile, which has the least halo.
-static TileShape auto_shape(int n, int cus) {
+TileShape cnv_auto_shape(int n, int cus) {
     const TileShape cand[] = {{5, 4, 6}, {5, 8, 4}, {6, 16, 4}, {5, 16, 4}};
     for (const TileShape& t : cand) {
         const long long tiles = (long long)((n + tile_rows(t) - 1) / tile_rows(t)) * ((n + tile_cols(t) - 1) / tile_cols(t));
@@ -326,7 +327,7 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || cus <= 0)
             cus = 256;
-        c->shape = auto_shape(p->n, cus);
+        c->shape = cnv_auto_shape(p->n, cus);
     }
     if (p->tile_waves > 0) c->shape.nw = p->tile_waves;
     if (p->wave_rows > 0) c->shape.rb = p->wave_rows;
@@ -423,7 +424,7 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) 
         *outer = o + 1;
         *fin = snap;
         fin->k = total;
-        if (snap.status == SOR_CONVERGED || snap.status == SOR_FAILED) return CNV_OK;
+        if (dc_final(snap.status)) return CNV_OK;
     }
     fin->status = SOR_FAILED;
     return CNV_OK;
@@ -452,6 +453,8 @@ static int capture(cnv_ctx* c, hipGraphExec_t* out, int (*body)(cnv_ctx*, int), 
 }
 
 static bool is_dc(const cnv_ctx* c) { return c->p.precision == CNV_PREC_FP32_DC; }
+// the continuity kernel also finds the largest |u| or |v|, which the blow-up check reads
+static bool wants_diag(const cnv_ctx* c) { return c->p.diagnostics || c->p.blowup_speed > 0; }
 
 // wall vorticity and transport of w[wc] into w[wc^1], then a fresh Poisson solve
 static int body_pre(cnv_ctx* c, int wc) {
@@ -516,7 +519,7 @@ static int body_correct(cnv_ctx* c, int) {
 static int body_post(cnv_ctx* c, int) {
     const double* p1 = is_dc(c) ? c->psi[0] : c->psi[1];
     LAUNCH(c, velocity_launch(c->psi[0], p1, c->ctl + 1, c->u, c->v, c->n, c->ld, c->st, c->stream));
-    if (c->p.diagnostics) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, c->n, c->ld, c->st, c->stream));
+    if (wants_diag(c)) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, c->n, c->ld, c->st, c->stream));
     return CNV_OK;
 }
 
@@ -603,13 +606,42 @@ static int poisson_dc_graph(cnv_ctx* c, int src, SorCtl* fin, int* outer) {
         *outer = o + 1;
         *fin = snap;
         fin->k = total;
-        if (snap.status == SOR_CONVERGED || snap.status == SOR_FAILED) return CNV_OK;
+        if (dc_final(snap.status)) return CNV_OK;
     }
     fin->status = SOR_FAILED;
     return CNV_OK;
 }
 
 static int one_step(cnv_ctx* c, cnv_step_stats* stats);
+
+// A step's statistics from its final solve slot and the diagnostics; the blow-up check.
+static int finish_step(cnv_ctx* c, const SorCtl& fin, int outer, cnv_step_stats* stats) {
+    cnv_step_stats s{};
+    s.status = solve_status(fin.status);
+    s.sweeps = fin.k;
+    s.k = fin.k - 1;
+    s.e = fin.e_last;
+    s.outer = outer;
+    double speed = 0.0;
+    if (wants_diag(c)) {
+        HIPC(c, hipMemcpyAsync(c->h_diag, c->diag, sizeof(Diag), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        s.cont_max = dkey_inv(c->h_diag->kmax);
+        s.cont_min = dkey_inv(c->h_diag->kmin);
+        speed = dkey_inv(c->h_diag->kspd);
+    }
+    if (s.status == CNV_OK && c->p.blowup_speed > 0 && !(speed <= c->p.blowup_speed)) s.status = CNV_DIVERGED;
+    if (stats) *stats = s;
+    if (s.status == CNV_DIVERGED) {
+        if (fin.status == SOR_DIVERGED)
+            return cnv_fail(c, CNV_DIVERGED, "Poisson solve diverged: non-finite L1 update after %d sweeps", fin.k);
+        return cnv_fail(c, CNV_DIVERGED, "blow-up: largest |u| or |v| %g exceeds blowup_speed %g", speed,
+                        c->p.blowup_speed);
+    }
+    if (s.status != CNV_OK)
+        return cnv_fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
+    return CNV_OK;
+}
 
 static int one_step_graph(cnv_ctx* c, cnv_step_stats* stats) {
     if (!c->g_post) {
@@ -627,22 +659,7 @@ static int one_step_graph(cnv_ctx* c, cnv_step_stats* stats) {
     if (rc) return rc;
     if (!is_dc(c)) c->k_prev = fin.k;
     HIPC(c, hipGraphLaunch(c->g_post, c->stream));
-    cnv_step_stats s{};
-    s.status = fin.status == SOR_CONVERGED ? CNV_OK : CNV_NOT_CONVERGED;
-    s.sweeps = fin.k;
-    s.k = fin.k - 1;
-    s.e = fin.e_last;
-    s.outer = outer;
-    if (c->p.diagnostics) {
-        HIPC(c, hipMemcpyAsync(c->h_diag, c->diag, sizeof(Diag), hipMemcpyDeviceToHost, c->stream));
-        HIPC(c, hipStreamSynchronize(c->stream));
-        s.cont_max = dkey_inv(c->h_diag->kmax);
-        s.cont_min = dkey_inv(c->h_diag->kmin);
-    }
-    if (stats) *stats = s;
-    if (s.status != CNV_OK)
-        return cnv_fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
-    return CNV_OK;
+    return finish_step(c, fin, outer, stats);
 }
 
 static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
@@ -668,23 +685,8 @@ static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
         LAUNCH(c, velocity_launch(c->psi[0], c->psi[0], c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
     else
         LAUNCH(c, velocity_launch(c->psi[0], c->psi[1], c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
-    if (c->p.diagnostics) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, n, ld, c->st, c->stream));
-    cnv_step_stats s{};
-    s.status = fin.status == SOR_CONVERGED ? CNV_OK : CNV_NOT_CONVERGED;
-    s.sweeps = fin.k;
-    s.k = fin.k - 1;
-    s.e = fin.e_last;
-    s.outer = outer;
-    if (c->p.diagnostics) {
-        HIPC(c, hipMemcpyAsync(c->h_diag, c->diag, sizeof(Diag), hipMemcpyDeviceToHost, c->stream));
-        HIPC(c, hipStreamSynchronize(c->stream));
-        s.cont_max = dkey_inv(c->h_diag->kmax);
-        s.cont_min = dkey_inv(c->h_diag->kmin);
-    }
-    if (stats) *stats = s;
-    if (s.status != CNV_OK)
-        return cnv_fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
-    return CNV_OK;
+    if (wants_diag(c)) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, n, ld, c->st, c->stream));
+    return finish_step(c, fin, outer, stats);
 }
 
 int cnv_step(cnv_ctx* c, int nsteps, cnv_step_stats* stats) {
@@ -725,6 +727,8 @@ int cnv_poisson_device(cnv_ctx* c, const double* d_f, double* d_psi, int* k, dou
     HIPC(c, hipStreamSynchronize(c->stream));
     if (k) *k = fin.k - 1;
     if (e) *e = fin.e_last;
+    if (fin.status == SOR_DIVERGED)
+        return cnv_fail(c, CNV_DIVERGED, "Poisson solve diverged: non-finite L1 update after %d sweeps", fin.k);
     if (fin.status != SOR_CONVERGED)
         return cnv_fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
     return CNV_OK;
@@ -744,6 +748,8 @@ int cnv_poisson_host(cnv_ctx* c, const double* f, double* psi, int* k, double* e
     HIPC(c, hipStreamSynchronize(c->stream));
     if (k) *k = fin.k - 1;
     if (e) *e = fin.e_last;
+    if (fin.status == SOR_DIVERGED)
+        return cnv_fail(c, CNV_DIVERGED, "Poisson solve diverged: non-finite L1 update after %d sweeps", fin.k);
     if (fin.status != SOR_CONVERGED)
         return cnv_fail(c, CNV_NOT_CONVERGED, "Error: maximum number of iterations achieved for Poisson equation.");
     return CNV_OK;

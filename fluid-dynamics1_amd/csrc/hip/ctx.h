@@ -94,28 +94,28 @@ inline int sor_vec2(const cnv::SorArgs<T>& a) {
     return (a.nc % 2 == 0) && (a.ld % 2 == 0) && !((uintptr_t)a.src_f & m) && !((uintptr_t)a.psi0 & m) && !((uintptr_t)a.psi1 & m);
 }
 
+// SOR launch arguments for an n x n grid (pitch ld) with spacing (dx, dy), relaxation beta, the
+// given tiling, scratch (partial sums, state slots) and planes src (source), b0 / b1 (ping-pong).
 template <typename T>
-inline cnv::SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int negate) {
+inline cnv::SorArgs<T> sor_args_for(int n, int ld, double dx, double dy, double beta, int tiles_i, int tiles_j,
+                                    double* partial, cnv::SorCtl* ctl, const T* src, T* b0, T* b1, int negate) {
     cnv::SorArgs<T> a{};
     a.src_f = src;
     a.psi0 = b0;
     a.psi1 = b1;
-    a.partial = c->partial;
-    a.ctl = c->ctl;
-    a.nr = c->n;
-    a.nc = c->n;
-    a.ld = c->ld;
-    a.tiles_i = c->tiles_i;
-    a.tiles_j = c->tiles_j;
+    a.partial = partial;
+    a.ctl = ctl;
+    a.nr = n;
+    a.nc = n;
+    a.ld = ld;
+    a.tiles_i = tiles_i;
+    a.tiles_j = tiles_j;
     a.negate = negate;
-    const double dx = c->p.dx, dy = c->p.dy;
     // the reference evaluates dy*dy, dx*dx, dx*dx*dy*dy and 2*(dx*dx+dy*dy) left to right
     a.c1 = (T)(dy * dy);
     a.c2 = (T)(dx * dx);
     a.c3 = (T)(dx * dx * dy * dy);
     a.B = (T)(2 * (dx * dx + dy * dy));
-    // FP32 planes (the defect-correction inner solve) may relax with their own factor
-    const double beta = (sizeof(T) == 4 && c->p.inner_beta > 0) ? c->p.inner_beta : c->p.beta;
     a.beta = (T)beta;
     a.omb = (T)(1 - beta);
     a.rB = (T)(1.0 / (2 * (dx * dx + dy * dy)));
@@ -125,6 +125,17 @@ inline cnv::SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int nega
     a.vec2 = sor_vec2(a);
     return a;
 }
+
+template <typename T>
+inline cnv::SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int negate) {
+    // FP32 planes (the defect-correction inner solve) may relax with their own factor
+    const double beta = (sizeof(T) == 4 && c->p.inner_beta > 0) ? c->p.inner_beta : c->p.beta;
+    return sor_args_for<T>(c->n, c->ld, c->p.dx, c->p.dy, beta, c->tiles_i, c->tiles_j, c->partial, c->ctl, src, b0,
+                           b1, negate);
+}
+
+// The default SOR launch shape of an n x n grid on a device with `cus` compute units.
+cnv::TileShape cnv_auto_shape(int n, int cus);
 
 // Queue SOR launches until the device control block reports convergence or failure.  Keeps
 // one chunk queued behind the one being waited on, so the GPU never idles on the host.
@@ -138,6 +149,15 @@ inline int launch_sor(cnv_ctx* c, cnv::SorArgs<float> a, int L, bool gs) {
 }
 
 inline bool sor_done(int status) { return status != cnv::SOR_RUNNING; }
+
+// A finished solve's slot status as a step status of cnv.h.
+inline int solve_status(int sor_status) {
+    return sor_status == cnv::SOR_CONVERGED ? CNV_OK : (sor_status == cnv::SOR_DIVERGED ? CNV_DIVERGED : CNV_NOT_CONVERGED);
+}
+// Ends the outer iterations of a defect-correction solve.
+inline bool dc_final(int sor_status) {
+    return sor_status == cnv::SOR_CONVERGED || sor_status == cnv::SOR_FAILED || sor_status == cnv::SOR_DIVERGED;
+}
 
 // Queue SOR launches (launch index L = 0, 1, ...) until the state slot written by a queued
 // launch reports a terminal status.  Keeps one chunk queued behind the one being waited on, so

@@ -62,11 +62,13 @@ __global__ void __launch_bounds__(kBlock) velocity_kernel(const double* psi0, co
 }
 
 // ---- continuity diagnostic: max/min of DX u + DY v (src/main.c:387-408, fluiddyn.c:149,
-//      linearalg.c:475-545; comparisons skip NaN exactly as "if (A > max)") ------------------
+//      linearalg.c:475-545; comparisons skip NaN exactly as "if (A > max)"), and the largest
+//      |u| or |v| (the blow-up check) --------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) continuity_kernel(const double* __restrict__ u,
                                                            const double* __restrict__ v, Diag* diag,
                                                            int n, int ld, Stencils s) {
     double mx = -DBL_MAX, mn = DBL_MAX;
+    unsigned long long ks = dkey(0.0);
     const long long count = (long long)n * n;
     const long long stride = (long long)gridDim.x * kBlock;
     for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) {
@@ -74,8 +76,11 @@ __global__ void __launch_bounds__(kBlock) continuity_kernel(const double* __rest
         const double c = continuity_at(u, v, ld, s, i, j);
         if (c > mx) mx = c;
         if (c < mn) mn = c;
+        const size_t p = (size_t)i * ld + j;
+        const unsigned long long kq = speed_key(u[p], v[p]);
+        ks = kq > ks ? kq : ks;
     }
-    diag_fold_block(mx, mn, diag);
+    diag_fold_block(mx, mn, ks, diag);
 }
 
 // ---- legacy elementwise entry points -------------------------------------------------------

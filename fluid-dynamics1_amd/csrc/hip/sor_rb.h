@@ -42,11 +42,14 @@
 //     final iterate is exactly the one of the first sweep with e < tol (src/poisson.c:273);
 //     FP32 planes, a preconditioner, test the last sweep of each launch only;
 //   * the sweep budget itmax is honoured exactly (src/poisson.c:234);
+//   * a non-finite L1 update (or iterate norm) ends the solve with SOR_DIVERGED;
 //   * relative rule: with rtol > 0 the solve also stops ("fold") once the update norm of a sweep
 //     fell below rtol times the iterate's own L1 norm after that launch's last sweep (measured
 //     once per launch) -- the resolution limit of an FP32 defect-correction inner solve, or the
 //     relative tolerance of the adjoint problem's FP64 solves (NORM).
 #pragma once
+
+#include <float.h>
 
 #include <type_traits>
 
@@ -477,15 +480,20 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         // the first sweep s < sd with e < tol (converged) or, failing that, below rtol times the
         // iterate's norm after the launch (fold); static indices keep etot in registers.  Sum i
         // belongs to sweep i (FP32: to the last sweep, sd - 1, so there is never a rollback).
-        int s_conv = -1, s_fold = -1;
+        // A sum that is NaN or +infinity (sums of |delta| are never negative) ends the solve as
+        // diverged at that sweep: no later sweep can bring a non-finite iterate below tol, and the
+        // reference's loop would spend its whole budget before failing (src/poisson.c:280-284).
+        int s_conv = -1, s_fold = -1, s_bad = -1;
 #pragma unroll
         for (int i = 0; i < SE; ++i) {
             const int s = kLastOnly ? sd - 1 : i;
-            if (s < sd && s_conv < 0 && s_fold < 0) {
-                if (etot[i] < tol) s_conv = s;
+            if (s < sd && s_conv < 0 && s_fold < 0 && s_bad < 0) {
+                if (!(etot[i] <= DBL_MAX)) s_bad = s;
+                else if (etot[i] < tol) s_conv = s;
                 else if (rtol > 0.0 && etot[i] < rtol * ntot) s_fold = s;
             }
         }
+        if (kNorm && s_conv < 0 && s_fold < 0 && s_bad < 0 && !(ntot <= DBL_MAX)) s_bad = sd - 1;
         const int s_rep = (s_conv >= 0 && s_conv < sd - 1) ? s_conv : sd - 1;  // reported sweep
         double e_rep = 0.0;
         const double n_rep = ntot;
@@ -511,6 +519,9 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
                 mode = 1;
             } else if (s_fold >= 0) {
                 next.status = SOR_FOLD;
+                mode = 1;
+            } else if (s_bad >= 0) {
+                next.status = SOR_DIVERGED;
                 mode = 1;
             } else if (next.k >= itmax) {
                 next.status = SOR_FAILED;

@@ -124,6 +124,7 @@ __device__ inline void ctl_reset(SorCtl* sl, double tol, double rtol, int itmax)
 __device__ inline void diag_reset(Diag* d) {
     d->kmax = dkey(-DBL_MAX);
     d->kmin = dkey(DBL_MAX);
+    d->kspd = dkey(0.0);
 }
 
 // ---- velocities u = DY psi, v = -(DX psi) at (i, j) (src/main.c:366-383) -------------------
@@ -141,31 +142,44 @@ __device__ inline double continuity_at(const double* u, const double* v, int ld,
     return sum_x(s.d1x, s.start, u, i, j, ld) + sum_y(s.d1y, s.start, v, i, j, ld);
 }
 
+// The speed key of a point: the larger key of |u| and |v|.  fabs clears the sign of a NaN, whose
+// key then orders above +infinity's, so a NaN velocity wins every max.
+__device__ inline unsigned long long speed_key(double uu, double vv) {
+    const unsigned long long a = dkey(fabs(uu)), b = dkey(fabs(vv));
+    return a > b ? a : b;
+}
+
 // Block-wide max/min of per-thread extrema as order-preserving keys, folded into *diag with
-// atomics (exact and order independent).  kBlock threads.
-__device__ inline void diag_fold_block(double mx, double mn, Diag* diag) {
+// atomics (exact and order independent), with the largest speed key.  kBlock threads.
+__device__ inline void diag_fold_block(double mx, double mn, unsigned long long ks, Diag* diag) {
     unsigned long long kx = dkey(mx), kn = dkey(mn);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long ox = __shfl_xor(kx, off, 64), on = __shfl_xor(kn, off, 64);
+        const unsigned long long os = __shfl_xor(ks, off, 64);
         kx = ox > kx ? ox : kx;
         kn = on < kn ? on : kn;
+        ks = os > ks ? os : ks;
     }
-    __shared__ unsigned long long sx[kBlock / 64], sn[kBlock / 64];
+    __shared__ unsigned long long sx[kBlock / 64], sn[kBlock / 64], ss[kBlock / 64];
     if ((threadIdx.x & 63) == 0) {
         sx[threadIdx.x >> 6] = kx;
         sn[threadIdx.x >> 6] = kn;
+        ss[threadIdx.x >> 6] = ks;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         kx = sx[0];
         kn = sn[0];
+        ks = ss[0];
         for (int w = 1; w < kBlock / 64; ++w) {
             kx = sx[w] > kx ? sx[w] : kx;
             kn = sn[w] < kn ? sn[w] : kn;
+            ks = ss[w] > ks ? ss[w] : ks;
         }
         atomicMax(&diag->kmax, kx);
         atomicMin(&diag->kmin, kn);
+        atomicMax(&diag->kspd, ks);
     }
 }
 

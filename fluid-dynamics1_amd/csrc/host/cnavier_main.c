@@ -252,7 +252,9 @@ int main(int argc, char *argv[])
         const double t0 = now_s();
         cnv_step_stats st;
         rc = cnv_step(ctx, 1, &st);
-        if (rc == CNV_NOT_CONVERGED) {
+        /* a non-finite Poisson update (CNV_DIVERGED) fails the way the reference's solve fails at
+           itmax (src/poisson.c:280-284), with the same message */
+        if (rc == CNV_NOT_CONVERGED || rc == CNV_DIVERGED) {
             logf_(log_file, "Error: maximum number of iterations achieved for Poisson equation.\n");
             printf("Error: maximum number of iterations achieved for Poisson equation.\n");
             vtk_writer_join(&writer);

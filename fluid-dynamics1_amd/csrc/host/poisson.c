@@ -69,7 +69,9 @@ static mtrx solve(mtrx f, double dx, double dy, int itmax, double tol, double be
     int k = 0;
     double e = 0;
     const int rc = cnv_poisson_host(g_ctx, flat, psi, &k, &e);
-    if (rc == CNV_NOT_CONVERGED) {
+    /* a non-finite update ends the GPU solve early (CNV_DIVERGED); the reference's loop would
+       spend itmax sweeps on it and then fail the same way (src/poisson.c:280-284) */
+    if (rc == CNV_NOT_CONVERGED || rc == CNV_DIVERGED) {
         if (out) {
             fprintf(out, "Error: maximum number of iterations achieved for Poisson equation.\n");
             fflush(out);

@@ -28,6 +28,10 @@ extern "C" {
 #define CNV_NOT_CONVERGED 3    /* Poisson hit itmax (src/poisson.c:280-284 would exit(1))   */
 #define CNV_ERR_NOMEM 4
 #define CNV_ERR_NO_DEVICE 5
+#define CNV_DIVERGED 6         /* a Poisson solve's L1 update became NaN or infinite, or a step's
+                                  largest |u| or |v| exceeded blowup_speed (or was NaN); the
+                                  reference has no such check: its solve runs to itmax and exits
+                                  (src/poisson.c:280-284)                                     */
 
 #define CNV_PREC_FP64 0        /* plain FP64, bit-faithful red-black SOR                      */
 #define CNV_PREC_FP32_DC 1     /* FP32 stencils and SOR sweeps inside FP64 defect correction */
@@ -76,10 +80,14 @@ typedef struct {
                               stencils the refined trajectory drifts from the FP64 one by their
                               rounding, step after step (2e-6 relative after 600 steps at 256^2,
                               Re 1000), against 1e-10 with FP64 transport                     */
+    double blowup_speed;   /* > 0: a step after which some |u| or |v| exceeds this (or is NaN)
+                              ends with CNV_DIVERGED (the explicit scheme has gone unstable);
+                              0 = off, as in the reference.  In a batch the cavity is dropped
+                              from later steps and the others go on                           */
 } cnv_params;
 
 typedef struct {
-    int status;            /* CNV_OK or CNV_NOT_CONVERGED                                    */
+    int status;            /* CNV_OK, CNV_NOT_CONVERGED or CNV_DIVERGED                      */
     int k;                 /* logged Poisson index = sweeps - 1 (src/poisson.c:275)          */
     int sweeps;            /* SOR sweeps of this step (FP64) / inner FP32 sweeps (DC)        */
     int outer;             /* defect-correction iterations (DC), 0 for FP64                  */
@@ -106,7 +114,9 @@ int cnv_set_state(cnv_ctx *ctx, const double *w, const double *psi, const double
 int cnv_get_state(cnv_ctx *ctx, double *w, double *psi, double *u, double *v);
 
 /* Advance nsteps time steps (src/main.c:276-451).  stats: nsteps entries or NULL.  Stops at the
- * first step whose Poisson solve does not converge and returns CNV_NOT_CONVERGED. */
+ * first step whose Poisson solve does not converge and returns CNV_NOT_CONVERGED, or at the first
+ * step that diverged and returns CNV_DIVERGED (the state then holds that step's non-finite or
+ * blown-up fields). */
 int cnv_step(cnv_ctx *ctx, int nsteps, cnv_step_stats *stats);
 
 /* Timed variant: wall time of the step loop measured with HIP events on the context stream. */
@@ -176,11 +186,21 @@ int cnv_batch_create(const cnv_params *items, int count, cnv_batch **out);
 void cnv_batch_destroy(cnv_batch *b);
 int cnv_batch_size(const cnv_batch *b);
 const char *cnv_batch_last_error(const cnv_batch *b);   /* b may be NULL: this thread's last batch error */
-/* Advance every cavity by nsteps.  stats: NULL or count*nsteps entries, stats[t*count + i]. */
+/* Advance every live cavity by nsteps.  stats: NULL or count*nsteps entries, stats[t*count + i].
+ * A cavity that diverges (CNV_DIVERGED: a non-finite Poisson update, or its blowup_speed exceeded)
+ * is reported in that step's stats, keeps its last fields and is left out of every later step
+ * (stats status CNV_DIVERGED, 0 sweeps); the others go on and the call returns CNV_OK.  A live
+ * cavity whose solve runs out of sweeps ends the call with CNV_NOT_CONVERGED. */
 int cnv_batch_step(cnv_batch *b, int nsteps, cnv_step_stats *stats);
 int cnv_batch_step_timed(cnv_batch *b, int nsteps, cnv_step_stats *stats, double *elapsed_ms);
 /* State of cavity i (host arrays of n_i*n_i doubles; NULL pointers skip a field). */
 int cnv_batch_get_state(cnv_batch *b, int i, double *w, double *psi, double *u, double *v);
+/* Replaces cavity i's state (NULL pointers keep a field); a diverged cavity becomes live again. */
+int cnv_batch_set_state(cnv_batch *b, int i, const double *w, const double *psi, const double *u,
+                        const double *v);
+/* Per-cavity status: status[i] = CNV_OK (live) or CNV_DIVERGED.  Returns the number of live
+ * cavities, or a negative CNV_ERR_* code. */
+int cnv_batch_status(const cnv_batch *b, int *status);
 /* Launch-shape autotuner: times `launches` batched SOR launches of every compiled shape on a
  * synthetic source over the whole batch, keeps the fastest per sweep and reports it.  Leaves w,
  * u, v as they were; the Poisson iterates are zeroed (every step recomputes psi from zero). */
@@ -206,7 +226,8 @@ void cnv_adj_destroy(cnv_adjoint *a);
 const char *cnv_adj_last_error(const cnv_adjoint *a);   /* a may be NULL: this thread's last adjoint error */
 /* Forward run from (w0, U) for nsteps; stores the checkpoints; returns E. */
 int cnv_adj_forward(cnv_adjoint *a, const double *w0, double U, int nsteps, double *energy);
-/* Gradient of the last forward run's E: dE_dw0 (host, n*n) and dE_dU (either may be NULL). */
+/* Gradient of the last forward run's E: dE_dw0 (host, n*n) and dE_dU (either may be NULL); repeated
+ * calls after one forward run return the same gradient. */
 int cnv_adj_backward(cnv_adjoint *a, double *dE_dw0, double *dE_dU);
 /* SOR sweeps of the last forward run and of the last backward run's adjoint solves. */
 int cnv_adj_sweeps(const cnv_adjoint *a, long long *forward, long long *adjoint);

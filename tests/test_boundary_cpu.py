@@ -142,7 +142,18 @@ def test_batch_rejects_mixed_precision_without_gpu():
     assert ei.value.code == cnv.ERR_ARG and "must match item 0" in str(ei.value)
 
 
+def _make_tables():
+    import importlib.util
+    import cnv
+    spec = importlib.util.spec_from_file_location("make_tables", os.path.join(cnv.WORKLOADS, "make_tables.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
 def test_batch_workload_table_is_seed0_draw():
+    # the committed table (what cnv.batch_table() and bench.py read) is numpy's seed-0 draw, value
+    # for value, and follows the workload's rules
     import cnv
     rows = cnv.batch_table()
     assert len(rows) == 64 and {r["n"] for r in rows} <= {64, 96, 128, 192, 256, 384, 512}
@@ -151,7 +162,25 @@ def test_batch_workload_table_is_seed0_draw():
         h = 1.0 / r["n"]
         assert r["dt"] == 0.2 * min(h * h * r["Re"] / 4.0, h)
         assert r["beta"] == cnv.reference_beta(r["n"])
-    assert rows == cnv.batch_table()
+    assert rows == _make_tables().draw_batch(64, 0)
+    assert cnv.batch_table(65, 0)[:64] != rows  # another count is a fresh draw, not the table
+
+
+def test_gaussian_table_is_seed1_draw():
+    import numpy as np
+    import cnv
+    cx, cy, width, amp = cnv.gaussian_parameters()
+    drawn = _make_tables().draw_gaussians(32, 1)
+    for got, key in zip((cx, cy, width, amp), ("cx", "cy", "width", "amplitude")):
+        assert np.array_equal(got, np.array(drawn[key]))
+    assert np.all((0 <= cx) & (cx < 1) & (0 <= cy) & (cy < 1)) and np.all((0.02 <= width) & (width < 0.1))
+    # the field at the nodes, from the table: a Gaussian's peak sits where its centre's node is
+    w = cnv.gaussian_vorticity(64)
+    x = np.arange(64) / 64
+    k = int(np.argmax(np.abs(amp)))
+    ref = sum(amp[m] * np.exp(-((x[None, :] - cx[m]) ** 2 + (x[:, None] - cy[m]) ** 2) / (2 * width[m] ** 2))
+              for m in range(32))
+    assert np.allclose(w, ref, rtol=0, atol=1e-12) and abs(w).max() > 0.5 * abs(amp[k])
 
 
 def test_params_layout_matches_library():

@@ -96,3 +96,43 @@ def test_gpu_adjoint_through_torch_autograd():
     assert np.max(np.abs(w0.grad.numpy() - g_ref)) <= 1e-8 * np.max(np.abs(g_ref))
     assert abs(float(U.grad) - dU_ref) <= 1e-8 * abs(dU_ref)
     adj.close()
+
+
+def test_gpu_adjoint_backward_twice():
+    # the backward pass recomputes segments from the checkpoints, which moves the context's own
+    # state; a second call after the same forward run must still seed from that run's final state
+    n, steps, U, Re, dt = 20, 7, 1.0, 250.0, 4e-3
+    w0 = cnv.gaussian_vorticity(n, count=6, seed=4)
+    with make(n, Re, dt, K=3) as adj:
+        adj.forward(w0, U, steps)
+        g1, dU1 = adj.backward()
+        g2, dU2 = adj.backward()
+    assert dU1 == dU2 and np.array_equal(g1, g2)
+
+
+def test_gpu_adjoint_workload_full_size():
+    # the adjoint workload itself: 256^2, Re 400, dt = 20*2^-16, 2000 steps, w0 = the committed
+    # 32 Gaussians (seed 1), lid U = 1, checkpoints every 100 steps; central differences
+    # (eps = 1e-6) along one random direction and in U within 1e-5, and the same gradient with
+    # checkpoints every 50 steps
+    n, steps, U, Re, dt = 256, 2000, 1.0, 400.0, 20 * 2.0 ** -16
+    w0 = cnv.gaussian_vorticity(n)
+    d = np.random.default_rng(11).standard_normal((n, n))
+    eps = 1e-6
+    with make(n, Re, dt, K=100) as adj:
+        E = adj.forward(w0, U, steps)
+        print("adjoint workload: E", E, flush=True)
+        g, dU = adj.backward()
+        print("adjoint workload: gradient done, dE/dU", dU, flush=True)
+        fd = (adj.forward(w0 + eps * d, U, steps) - adj.forward(w0 - eps * d, U, steps)) / (2 * eps)
+        print("adjoint workload: direction", float(np.sum(g * d)), fd, flush=True)
+        fdU = (adj.forward(w0, U + eps, steps) - adj.forward(w0, U - eps, steps)) / (2 * eps)
+        print("adjoint workload: lid", dU, fdU, flush=True)
+    assert abs(np.sum(g * d) - fd) <= 1e-5 * abs(fd)
+    assert abs(dU - fdU) <= 1e-5 * abs(fdU)
+    assert np.all(g[walls(n)] == 0.0)
+    with make(n, Re, dt, K=50) as adj:
+        assert adj.forward(w0, U, steps) == E
+        g50, dU50 = adj.backward()
+    print("adjoint workload: K=50 done", flush=True)
+    assert dU50 == dU and np.array_equal(g50, g)
