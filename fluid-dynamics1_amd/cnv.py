@@ -77,6 +77,9 @@ def lib():
         L.cnv_poisson_host.argtypes = [C.c_void_p, _DP, _DP, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.cnv_poisson_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int),
                                          C.POINTER(C.c_double)]
+        L.cnv_poisson_device_noalloc.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                                 C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int),
+                                                 C.POINTER(C.c_double)]
         L.cnv_euler_host.argtypes = [C.c_int] + [_DP] * 7 + [C.c_double, C.c_double]
         L.cnv_add_host.argtypes = [C.c_int, C.c_int, _DP, _DP, _DP]
         L.cnv_sub_host.argtypes = [C.c_int, C.c_int, _DP, _DP, _DP]
