@@ -124,11 +124,11 @@ __global__ void __launch_bounds__(kBlock) adj_vel_src_kernel(const double* lu, c
 }
 
 // lw += c3 z on the interior (z: the adjoint solve's final iterate)
-__global__ void __launch_bounds__(kBlock) adj_accum_kernel(double* lw, const double* z0, const double* z1,
-                                                          const SorCtl* fin, int n, int ld, double c3) {
+__global__ void __launch_bounds__(kBlock) adj_accum_kernel(double* lw, Planes zp, const SorCtl* fin, int n, int ld,
+                                                          double c3) {
     int i, j;
     if (!cell(n, i, j) || !interior(n, i, j)) return;
-    const double* z = fin->parity ? z1 : z0;
+    const double* z = zp.p[fin->parity];
     const size_t q = (size_t)i * ld + j;
     lw[q] += c3 * z[q];
 }
@@ -251,7 +251,7 @@ static int solve_rel(cnv_adjoint* a, const double* src, double rtol, int* kpred,
     ACTX(a, poisson_init_launch(c->psi[0], c->ctl, nullptr, c->n, c->ld, 0.0, c->p.itmax, c->stream));
     hipLaunchKernelGGL(adj_slots_kernel, dim3(1), dim3(64), 0, c->stream, c->ctl, 1e-300, rtol, c->p.itmax);
     ALAUNCH(a);
-    SorArgs<double> sa = sor_args<double>(c, src, c->psi[0], c->psi[1], 1);
+    SorArgs<double> sa = sor_args64(c, src, 1);
     sa.norm = 1;
     SorCtl fin{};
     c->k_prev = *kpred;
@@ -278,7 +278,7 @@ static int fwd_step(cnv_adjoint* a) {
     int rc = solve_rel(a, w_out, a->rtol_fwd, &a->k_fwd, &k);
     if (rc) return rc;
     a->sweeps_fwd += k;
-    ACTX(a, velocity_launch(c->psi[0], c->psi[1], c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
+    ACTX(a, velocity_launch(planes_of(c), c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
     return CNV_OK;
 }
 
@@ -314,7 +314,7 @@ static int rev_step(cnv_adjoint* a, int idx) {
     int rc = solve_rel(a, a->src, a->rtol_adj, &a->k_adj, &k);
     if (rc) return rc;
     a->sweeps_adj += k;
-    hipLaunchKernelGGL(adj_accum_kernel, g, blk, 0, c->stream, a->lw, c->psi[0], c->psi[1], c->ctl + c->fin_slot, n,
+    hipLaunchKernelGGL(adj_accum_kernel, g, blk, 0, c->stream, a->lw, planes_of(c), c->ctl + c->fin_slot, n,
                        ld, c3);
     ALAUNCH(a);
     const double* ws = slot3(a->d_seg, idx, 0, a->cells);

@@ -44,7 +44,7 @@ namespace {
 // Per-cavity description read by the batched kernels.
 struct StepItem {
     double* w[2];
-    double* psi[2];
+    double* psi[3];
     double* u;
     double* v;
     float* e32[2];
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kBlock) batch_velocity_kernel(const StepItem* 
     const StepItem& I = it[m.inst];
     int i, j;
     if (!cell_of(m, I.n, i, j)) return;
-    const double* psi = DC ? I.psi[0] : I.psi[I.ctl[fin_slot].parity & 1];
+    const double* psi = DC ? I.psi[0] : I.psi[I.ctl[fin_slot].parity];
     velocity_at(psi, I.u, I.v, I.ld, I.st, i, j);
 }
 
@@ -223,6 +223,7 @@ __global__ void __launch_bounds__(kBlock) batch_zero_kernel(const StepItem* __re
     for (long long q = (long long)m.lb * kBlock + threadIdx.x; q < count; q += (long long)m.nb * kBlock) {
         I.psi[0][q] = 0.0;
         I.psi[1][q] = 0.0;
+        I.psi[2][q] = 0.0;
         if (I.e32[0]) {
             I.e32[0][q] = 0.0f;
             I.e32[1][q] = 0.0f;
@@ -344,7 +345,7 @@ static int build_sor(cnv_batch* b, TileShape sh) {
     for (int i = 0; i < b->count; ++i) {
         const int nt = tiles_of(b->ctx[i]->n, sh, &ti[i], &tj[i]);
         if (b->alive[i])
-            for (int t = 0; t < nt; ++t) list.push_back({i, t});
+            for (int t = 0; t <= nt; ++t) list.push_back({i, t});  // t == nt: the cavity's decider
     }
     while (list.size() % 8) list.push_back({-1, 0});
     const int per = (int)list.size() / 8;
@@ -371,7 +372,7 @@ static int build_sor(cnv_batch* b, TileShape sh) {
     for (int i = 0; i < b->count; ++i) {
         cnv_ctx* c = b->ctx[i];
         for (int w = 0; w < 2; ++w) {
-            SorArgs<double> a = sor_args<double>(c, c->w[w], c->psi[0], c->psi[1], 1);
+            SorArgs<double> a = sor_args64(c, c->w[w], 1);
             a.ctl = b->d_ctl + 2 * i;
             a.partial = b->d_partial + b->part_off[i];
             a.tiles_i = ti[i];
@@ -379,7 +380,7 @@ static int build_sor(cnv_batch* b, TileShape sh) {
             a64[w].push_back(a);
         }
         if (b->dc) {
-            SorArgs<float> a = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
+            SorArgs<float> a = sor_args32(c);
             a.ctl = b->d_ctl + 2 * i;
             a.partial = b->d_partial + b->part_off[i];
             a.tiles_i = ti[i];
@@ -441,7 +442,7 @@ static int run_sor_batch(cnv_batch* b, bool fp32, std::vector<SorCtl>& snaps) {
             BHIP(b, hipEventSynchronize(b->ev[s0]));
             const SorCtl* h = b->h_ctl + s0 * per;
             bool all = true;
-            for (int i = 0; i < b->count && all; ++i) all = !b->alive[i] || h[2 * i + f0].status != SOR_RUNNING;
+            for (int i = 0; i < b->count && all; ++i) all = !b->alive[i] || sor_done(h[2 * i + f0].status);
             if (all) {
                 snaps.resize(b->count);
                 for (int i = 0; i < b->count; ++i) snaps[i] = h[2 * i + f0];
@@ -674,6 +675,7 @@ int cnv_batch_create(const cnv_params* items, int count, cnv_batch** out) {
             I.w[1] = c->w[1];
             I.psi[0] = c->psi[0];
             I.psi[1] = c->psi[1];
+            I.psi[2] = c->psi[2];
             I.u = c->u;
             I.v = c->v;
             I.e32[0] = c->e32[0];
@@ -748,7 +750,7 @@ int cnv_batch_get_state(cnv_batch* b, int i, double* w, double* psi, double* u, 
     // (c->fin_slot and c->wcur: as the cavity's last step left them, also after it diverged)
     SorCtl sl{};
     BHIP(b, hipMemcpy(&sl, b->d_ctl + 2 * i + c->fin_slot, sizeof(SorCtl), hipMemcpyDeviceToHost));
-    const double* psrc = b->dc ? c->psi[0] : c->psi[sl.parity & 1];
+    const double* psrc = b->dc ? c->psi[0] : c->psi[sl.parity];
     const size_t bytes = sizeof(double) * c->cells;
     if (psi) BHIP(b, hipMemcpy(psi, psrc, bytes, hipMemcpyDeviceToHost));
     if (w) BHIP(b, hipMemcpy(w, c->w[c->wcur], bytes, hipMemcpyDeviceToHost));

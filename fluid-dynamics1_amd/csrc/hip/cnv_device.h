@@ -9,28 +9,49 @@ namespace cnv {
 constexpr int kStencilWidth = 7;  // every Diff1/Diff2 row fits a 7-column window
 constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
 
-// Device-side state of one Poisson solve, kept in two slots: SOR launch L (launch index passed
-// by the host, counted from 0 per solve) reads slot (L-1)&1, decides on the previous launch's
-// per-tile partial sums, and writes slot L&1.  The init kernels write slot 1 for launch 0.
+// Device-side state of one Poisson solve, kept in two slots.  SOR launch L (launch index passed
+// by the host, counted from 0 per solve) has one deciding workgroup, which reads slot (L-1)&1 and
+// the previous launch's per-tile partial sums and writes slot L&1; its sweeping workgroups read
+// slot (L-1)&1 only (the state after launch L-2, and how many sweeps launch L-1 did).  The init
+// kernels write slot 1 for launch 0.
 enum SorStatus : int {
     SOR_RUNNING = 0,
     SOR_CONVERGED = 1,  // a sweep's L1 update fell below tol
     SOR_FAILED = 2,     // itmax sweeps without convergence
     SOR_FOLD = 4,       // defect correction: the FP32 iterate reached its resolution (rtol)
-    SOR_DIVERGED = 8    // a sweep's L1 update (or the iterate's norm) was NaN or infinite
+    SOR_DIVERGED = 8,   // a sweep's L1 update (or the iterate's norm) was NaN or infinite
+    SOR_ROLLBACK = 16   // converged inside a launch before its last sweep: the next launch redoes
+                        // rb_n sweeps from plane rb_src into plane `parity` (not terminal yet)
 };
 
 struct SorCtl {
     int k;              // sweeps accepted before the launch that wrote the slot
-    int parity;         // terminal states: the buffer holding the final iterate
+    int parity;         // terminal states (and ROLLBACK): the plane holding the final iterate
     int status;         // SorStatus after deciding on the previous launch
     int s_done;         // sweeps the writing launch performs itself (0: none)
     int launches;       // launches that did work (diagnostic)
     int itmax;          // sweep budget of this solve
+    int rb_src;         // ROLLBACK: the plane to sweep from (the input of the converged launch)
+    int rb_n;           // ROLLBACK: sweeps to redo
     double e_last;      // L1 update of the last accepted sweep
     double tol;         // stop when a sweep's L1 update < tol
     double rtol;        // stop ("fold") when the L1 update < rtol * L1 norm of the iterate (0: never)
     double n_last;      // L1 norm of the iterate after the last accepted sweep (FP32 planes)
+};
+static_assert(sizeof(SorCtl) == 64, "one state slot per 64 bytes");
+
+// Iterate planes a solve rotates through: launch L reads plane L % NB and writes (L+1) % NB.
+// FP64 three, so that the input of the launch before last is still intact when the decision on
+// it arrives (a rollback redoes sweeps from it); FP32 two (its stopping rule looks at each
+// launch's last sweep only, so it never rolls back).
+template <typename T>
+struct SorPlanes {
+    static constexpr int value = sizeof(T) == 8 ? 3 : 2;
+};
+
+// The planes of an FP64 solve, for the kernels that read its final iterate (plane fin->parity).
+struct Planes {
+    const double* p[3];
 };
 
 // Continuity diagnostic: max/min as order-preserving unsigned keys (exact, order independent),
@@ -73,9 +94,10 @@ struct WallBC {
 template <typename T>
 struct SorArgs {
     const T* src_f;      // source term: w (plain path, negate = 1) or the FP32 residual
-    T* psi0;
+    T* psi0;             // iterate planes (psi2: FP64 only)
     T* psi1;
-    double* partial;     // [2][kMaxSweepsPerLaunch][ntiles][2]: per-workgroup L1 update and
+    T* psi2;
+    double* partial;     // [2][kMaxSweepsPerLaunch][ntiles][2]: per-tile L1 update and
                          // iterate norm, double-buffered by launch parity
     SorCtl* ctl;         // the solve's two state slots
     int launch;          // launch index within the solve
@@ -88,7 +110,8 @@ struct SorArgs {
     int norm;            // FP64: also sum the iterate's L1 norm (needed when rtol > 0)
 };
 
-// Batched SOR launch: workgroup -> (solve, tile); inst < 0 marks padding workgroups.
+// Batched SOR launch: workgroup -> (solve, tile); inst < 0 marks padding workgroups, tile ==
+// the solve's tile count its deciding workgroup.
 struct SorBlock {
     int inst;
     int tile;
@@ -119,10 +142,10 @@ int transport_fp32_launch(const double* w_in, double* w_out, const double* u, co
 // zero = 0 keeps psi0 (warm start) and only resets the state slots and the diagnostic
 int poisson_init_launch(double* psi0, SorCtl* ctl, Diag* diag, int n, int ld, double tol,
                         int itmax, hipStream_t st, int zero = 1);
-// warm start: psi0 <- psi[fin->parity] (the previous solve's final iterate)
-int warm_start_launch(double* psi0, const double* psi1, const SorCtl* fin, int n, int ld, hipStream_t st);
-int velocity_launch(const double* psi0, const double* psi1, const SorCtl* fin, double* u,
-                    double* v, int n, int ld, const Stencils& s, hipStream_t st);
+// warm start: psi0 <- psi.p[fin->parity] (the previous solve's final iterate)
+int warm_start_launch(double* psi0, const Planes& psi, const SorCtl* fin, int n, int ld, hipStream_t st);
+int velocity_launch(const Planes& psi, const SorCtl* fin, double* u, double* v, int n, int ld,
+                    const Stencils& s, hipStream_t st);
 int continuity_launch(const double* u, const double* v, Diag* diag, int n, int ld,
                       const Stencils& s, hipStream_t st);
 int euler_launch(double* w, const double* dwdx, const double* dwdy, const double* d2x,
@@ -131,8 +154,8 @@ int euler_launch(double* w, const double* dwdx, const double* dwdy, const double
 int add_launch(const double* a, const double* b, double* out, long long count, int sub,
                hipStream_t st);
 int copy_launch(double* dst, const double* src, long long count, hipStream_t st);
-int snapshot_launch(const double* w, const double* psi0, const double* psi1, const double* u, const double* v,
-                    const SorCtl* fin, int psi_fixed, double* dst, long long count, hipStream_t st);
+int snapshot_launch(const double* w, const Planes& psi, const double* u, const double* v, const SorCtl* fin,
+                    double* dst, long long count, hipStream_t st);
 
 // defect correction
 int dc_residual_launch(const double* w, const double* psi, float* r32, int n, int ld, double c1,

@@ -165,9 +165,8 @@ int cnv_set_state(cnv_ctx* c, const double* w, const double* psi, const double* 
     const size_t bytes = sizeof(double) * c->cells;
     if (w) HIPC(c, hipMemcpyAsync(c->w[c->wcur], w, bytes, hipMemcpyHostToDevice, c->stream));
     if (psi) {
-        // psi lives in psi[parity]; make both buffers hold it so that any parity reads it
-        HIPC(c, hipMemcpyAsync(c->psi[0], psi, bytes, hipMemcpyHostToDevice, c->stream));
-        HIPC(c, hipMemcpyAsync(c->psi[1], psi, bytes, hipMemcpyHostToDevice, c->stream));
+        // psi lives in psi[parity]; make every plane hold it so that any parity reads it
+        for (int b = 0; b < 3; ++b) HIPC(c, hipMemcpyAsync(c->psi[b], psi, bytes, hipMemcpyHostToDevice, c->stream));
     }
     if (u) HIPC(c, hipMemcpyAsync(c->u, u, bytes, hipMemcpyHostToDevice, c->stream));
     if (v) HIPC(c, hipMemcpyAsync(c->v, v, bytes, hipMemcpyHostToDevice, c->stream));
@@ -182,7 +181,7 @@ static int current_psi(cnv_ctx* c, const double** out) {
     }
     HIPC(c, hipMemcpyAsync(c->h_ctl, c->ctl + c->fin_slot, sizeof(SorCtl), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
-    *out = c->psi[c->h_ctl[0].parity & 1];  // parity of the final iterate
+    *out = c->psi[c->h_ctl[0].parity];  // the plane of the final iterate
     return CNV_OK;
 }
 
@@ -215,8 +214,7 @@ int cnv_snapshot_begin(cnv_ctx* c) {
     }
     // the previous snapshot's transfer reads the device copy that is about to be overwritten
     if (c->snap_begun) HIPC(c, hipEventSynchronize(c->ev_snap_done));
-    const bool fixed = c->p.precision == CNV_PREC_FP32_DC;  // the refined form's psi is psi[0]
-    LAUNCH(c, snapshot_launch(c->w[c->wcur], c->psi[0], c->psi[1], c->u, c->v, c->ctl + c->fin_slot, fixed,
+    LAUNCH(c, snapshot_launch(c->w[c->wcur], planes_of(c), c->u, c->v, c->ctl + c->fin_slot,
                               c->d_snap, (long long)c->cells, c->stream));
     HIPC(c, hipEventRecord(c->ev_snap_ready, c->stream));
     HIPC(c, hipStreamWaitEvent(c->snap_stream, c->ev_snap_ready, 0));
@@ -280,7 +278,7 @@ void cnv_destroy(cnv_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->snap_stream) (void)hipStreamSynchronize(c->snap_stream);
     drop_graphs(c);
-    void* dev[] = {c->w[0], c->w[1], c->psi[0], c->psi[1], c->u, c->v, c->d_start, c->d_tab, c->d_ftab,
+    void* dev[] = {c->w[0], c->w[1], c->psi[0], c->psi[1], c->psi[2], c->u, c->v, c->d_start, c->d_tab, c->d_ftab,
                    c->ctl, c->diag, c->partial, c->e32[0], c->e32[1], c->r32, c->d_budget, c->d_snap};
     for (void* p : dev)
         if (p) (void)hipFree(p);
@@ -336,20 +334,19 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
         return cnv_fail(nullptr, CNV_ERR_ARG, "unsupported SOR launch shape (sweeps_per_launch %d, waves %d, rows per wave %d)",
                     spl, p->tile_waves, p->wave_rows);
     }
-    // captured chunks of 64 launches: each replay costs a graph launch and a poll copy in the
-    // stream, and a solve's end up to two chunks of no-op launches; at 1024^2 64 measured 0.6%
-    // faster per step than 32 or 128.  Eager solves choose their chunk per solve (run_sor).
-    c->launches_per_poll = p->launches_per_poll > 0 ? p->launches_per_poll : 64;
+    // captured chunks of 66 launches: each replay costs a graph launch and a poll copy in the
+    // stream, and a solve's end up to two chunks of no-op launches; at 1024^2 chunks of 64
+    // measured 0.6% faster per step than 32 or 128 (the chunk length is a multiple of 6, see
+    // build_graphs).  Eager solves choose their chunk per solve (run_sor).
+    c->launches_per_poll = p->launches_per_poll > 0 ? p->launches_per_poll : 66;
     c->poll_fixed = p->launches_per_poll > 0 ? p->launches_per_poll : 0;
     c->tiles_i = (c->n + tile_rows(c->shape) - 1) / tile_rows(c->shape);
     c->tiles_j = (c->n + tile_cols(c->shape) - 1) / tile_cols(c->shape);
     auto setup = [&]() -> int {
         HIPC(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         const size_t bytes = sizeof(double) * c->cells;
-        for (int b = 0; b < 2; ++b) {
-            HIPC(c, hipMalloc(&c->w[b], bytes));
-            HIPC(c, hipMalloc(&c->psi[b], bytes));
-        }
+        for (int b = 0; b < 2; ++b) HIPC(c, hipMalloc(&c->w[b], bytes));
+        for (int b = 0; b < 3; ++b) HIPC(c, hipMalloc(&c->psi[b], bytes));
         HIPC(c, hipMalloc(&c->u, bytes));
         HIPC(c, hipMalloc(&c->v, bytes));
         HIPC(c, hipMalloc(&c->ctl, 2 * sizeof(SorCtl)));
@@ -388,10 +385,10 @@ int cnv_sweeps_per_launch(const cnv_ctx* c) { return c ? c->shape.s : 0; }
 // ---- SOR driver ---------------------------------------------------------------------------------
 static int poisson_fp64(cnv_ctx* c, const double* f_src, int negate, SorCtl* fin) {
     if (c->p.warm_start)
-        LAUNCH(c, warm_start_launch(c->psi[0], c->psi[1], c->ctl + c->fin_slot, c->n, c->ld, c->stream));
+        LAUNCH(c, warm_start_launch(c->psi[0], planes_of(c), c->ctl + c->fin_slot, c->n, c->ld, c->stream));
     LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, c->p.tol, c->p.itmax, c->stream,
                                   !c->p.warm_start));
-    SorArgs<double> a = sor_args<double>(c, f_src, c->psi[0], c->psi[1], negate);
+    SorArgs<double> a = sor_args64(c, f_src, negate);
     int rc = run_sor<double>(c, a, fin);
     if (rc) return rc;
     c->k_prev = fin->k;
@@ -405,7 +402,7 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) 
                                   !c->p.warm_start));
     const double dx = c->p.dx, dy = c->p.dy;
     const double c1 = dy * dy, c2 = dx * dx, c3 = dx * dx * dy * dy, B = 2 * (dx * dx + dy * dy);
-    SorArgs<float> a = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
+    SorArgs<float> a = sor_args32(c);
     int total = 0;
     const int max_outer = c->p.max_outer > 0 ? c->p.max_outer : 50;
     *fin = SorCtl{};
@@ -431,10 +428,10 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) 
 }
 
 // ---- captured step (use_graph) -------------------------------------------------------------
-// Launch index L only matters through its parity and every solve's state lives in the device
-// slots, so a captured chunk of M (even) SOR launches with L = 0..M-1 is replayed until the
-// solve ends: each replay continues the previous one's slot parity, and after whole chunks the
-// final slot is 1.  Two chunk sizes: launches_per_poll for the predicted length of the solve
+// Launch index L only matters through L mod 2 (state slots, partial sums) and L mod 3 (FP64
+// iterate planes), and every solve's state lives in the device slots, so a captured chunk of M
+// (a multiple of 6) SOR launches with L = 0..M-1 is replayed until the solve ends: each replay
+// continues the previous one's rotation, and after whole chunks the final slot is 1.  Two chunk sizes: launches_per_poll for the predicted length of the solve
 // and kTailChunk for its end, so that the no-op launches queued after convergence stay few.
 static int capture(cnv_ctx* c, hipGraphExec_t* out, int (*body)(cnv_ctx*, int), int arg) {
     hipGraph_t g = nullptr;
@@ -476,7 +473,7 @@ static int body_pre(cnv_ctx* c, int wc) {
     LAUNCH(c, transport_fp64_launch(c->w[wc], c->w[wc ^ 1], c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re, c->p.dt,
                                     c->stream));
     // warm start: whole chunks end on slot 1, which holds the previous solve's final state
-    if (c->p.warm_start) LAUNCH(c, warm_start_launch(c->psi[0], c->psi[1], c->ctl + 1, n, ld, c->stream));
+    if (c->p.warm_start) LAUNCH(c, warm_start_launch(c->psi[0], planes_of(c), c->ctl + 1, n, ld, c->stream));
     LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, n, ld, c->p.tol, c->p.itmax, c->stream,
                                   !c->p.warm_start));
     return CNV_OK;
@@ -486,12 +483,12 @@ static int body_pre(cnv_ctx* c, int wc) {
 static int body_chunk_of(cnv_ctx* c, int src, int m) {
     const bool gs = c->p.poisson_type == 1;
     if (is_dc(c)) {
-        const SorArgs<float> a = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
+        const SorArgs<float> a = sor_args32(c);
         for (int l = 0; l < m; ++l)
             if (launch_sor(c, a, l, gs)) return cnv_fail(c, CNV_ERR_HIP, "SOR launch failed while capturing");
         return CNV_OK;
     }
-    const SorArgs<double> a = sor_args<double>(c, c->w[src], c->psi[0], c->psi[1], 1);
+    const SorArgs<double> a = sor_args64(c, c->w[src], 1);
     for (int l = 0; l < m; ++l)
         if (launch_sor(c, a, l, gs)) return cnv_fail(c, CNV_ERR_HIP, "SOR launch failed while capturing");
     return CNV_OK;
@@ -517,14 +514,14 @@ static int body_correct(cnv_ctx* c, int) {
 // velocities from the final iterate (slot 1 after whole chunks; psi[0] in the refined form),
 // continuity diagnostic
 static int body_post(cnv_ctx* c, int) {
-    const double* p1 = is_dc(c) ? c->psi[0] : c->psi[1];
-    LAUNCH(c, velocity_launch(c->psi[0], p1, c->ctl + 1, c->u, c->v, c->n, c->ld, c->st, c->stream));
+    LAUNCH(c, velocity_launch(planes_of(c), c->ctl + 1, c->u, c->v, c->n, c->ld, c->st, c->stream));
     if (wants_diag(c)) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, c->n, c->ld, c->st, c->stream));
     return CNV_OK;
 }
 
 static int build_graphs(cnv_ctx* c) {
-    if (c->launches_per_poll & 1) ++c->launches_per_poll;  // whole chunks end on slot 1
+    // whole chunks end on slot 1 and on the plane rotation's phase (launch index mod 2 and mod 3)
+    c->launches_per_poll = (c->launches_per_poll + 5) / 6 * 6;
     const int nsrc = is_dc(c) ? 1 : 2;  // the refined form's inner solves always read r32
     for (int wc = 0; wc < 2; ++wc) {
         int rc = capture(c, &c->g_pre[wc], body_pre, wc);
@@ -682,9 +679,9 @@ static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
     // velocities in FP64 in both forms: v = -(DX psi) takes small differences of a large psi,
     // which FP32 stencils would resolve only to ~1e-7 of |psi|/h
     if (dc)
-        LAUNCH(c, velocity_launch(c->psi[0], c->psi[0], c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
+        LAUNCH(c, velocity_launch(planes_of(c), c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
     else
-        LAUNCH(c, velocity_launch(c->psi[0], c->psi[1], c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
+        LAUNCH(c, velocity_launch(planes_of(c), c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
     if (wants_diag(c)) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, n, ld, c->st, c->stream));
     return finish_step(c, fin, outer, stats);
 }
@@ -723,7 +720,7 @@ int cnv_poisson_device(cnv_ctx* c, const double* d_f, double* d_psi, int* k, dou
     SorCtl fin{};
     int rc = poisson_fp64(c, d_f, 0, &fin);
     if (rc) return rc;
-    HIPC(c, hipMemcpyAsync(d_psi, c->psi[fin.parity & 1], sizeof(double) * c->cells, hipMemcpyDeviceToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(d_psi, c->psi[fin.parity], sizeof(double) * c->cells, hipMemcpyDeviceToDevice, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     if (k) *k = fin.k - 1;
     if (e) *e = fin.e_last;
@@ -744,7 +741,7 @@ int cnv_poisson_host(cnv_ctx* c, const double* f, double* psi, int* k, double* e
     SorCtl fin{};
     int rc = poisson_fp64(c, d_f, 0, &fin);
     if (rc) return rc;
-    HIPC(c, hipMemcpyAsync(psi, c->psi[fin.parity & 1], bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(psi, c->psi[fin.parity], bytes, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     if (k) *k = fin.k - 1;
     if (e) *e = fin.e_last;
@@ -768,8 +765,8 @@ int cnv_time_sor(cnv_ctx* c, int launches, double* mean_ms) {
     } else {
         LAUNCH(c, poisson_init_launch(c->psi[0], c->ctl, c->diag, c->n, c->ld, -1.0, 0x7fffffff, c->stream));
     }
-    SorArgs<double> a64 = sor_args<double>(c, c->w[c->wcur], c->psi[0], c->psi[1], 1);
-    SorArgs<float> a32 = sor_args<float>(c, c->r32, c->e32[0], c->e32[1], 0);
+    SorArgs<double> a64 = sor_args64(c, c->w[c->wcur], 1);
+    SorArgs<float> a32 = sor_args32(c);
     const bool gs = c->p.poisson_type == 1;
     // Time back-to-back launches (each decides on its predecessor, as in a solve) in the steady
     // state of the solve: its first few hundred launches, while the iterate spreads out from

@@ -20,7 +20,7 @@ struct cnv_ctx {
     // state
     double* w[2] = {nullptr, nullptr};
     int wcur = 0;
-    double* psi[2] = {nullptr, nullptr};
+    double* psi[3] = {nullptr, nullptr, nullptr};  // FP64 SOR planes; FP32_DC: psi[0] accumulates
     double* u = nullptr;
     double* v = nullptr;
     // tables
@@ -43,7 +43,7 @@ struct cnv_ctx {
     // launch shape
     cnv::TileShape shape{5, 16, 4};
     int tiles_i = 0, tiles_j = 0;
-    int launches_per_poll = 64;  // SOR launches per captured chunk (graph path)
+    int launches_per_poll = 66;  // SOR launches per captured chunk (graph path), a multiple of 6
     int poll_fixed = 0;   // launches per poll as requested; 0: chosen per solve (eager path)
     int k_prev = 0;       // sweeps the next solve is predicted to take (the previous one's)
     static constexpr int kDcPred = 64;
@@ -53,7 +53,7 @@ struct cnv_ctx {
     // SOR launches and a tail chunk of kTailChunk per source plane (FP64; the refined form's
     // inner solves use index 0), post-solve launches; the refined form adds, per outer
     // iteration, a residual + inner-init graph per w parity and a correction graph
-    static constexpr int kTailChunk = 4;
+    static constexpr int kTailChunk = 6;
     hipGraphExec_t g_pre[2] = {nullptr, nullptr};
     hipGraphExec_t g_chunk[2] = {nullptr, nullptr};
     hipGraphExec_t g_tail[2] = {nullptr, nullptr};
@@ -95,14 +95,17 @@ inline int sor_vec2(const cnv::SorArgs<T>& a) {
 }
 
 // SOR launch arguments for an n x n grid (pitch ld) with spacing (dx, dy), relaxation beta, the
-// given tiling, scratch (partial sums, state slots) and planes src (source), b0 / b1 (ping-pong).
+// given tiling, scratch (partial sums, state slots), source plane src and iterate planes b0, b1,
+// b2 (b2: FP64 only, nullptr for FP32).
 template <typename T>
 inline cnv::SorArgs<T> sor_args_for(int n, int ld, double dx, double dy, double beta, int tiles_i, int tiles_j,
-                                    double* partial, cnv::SorCtl* ctl, const T* src, T* b0, T* b1, int negate) {
+                                    double* partial, cnv::SorCtl* ctl, const T* src, T* b0, T* b1, T* b2,
+                                    int negate) {
     cnv::SorArgs<T> a{};
     a.src_f = src;
     a.psi0 = b0;
     a.psi1 = b1;
+    a.psi2 = b2;
     a.partial = partial;
     a.ctl = ctl;
     a.nr = n;
@@ -126,12 +129,23 @@ inline cnv::SorArgs<T> sor_args_for(int n, int ld, double dx, double dy, double 
     return a;
 }
 
-template <typename T>
-inline cnv::SorArgs<T> sor_args(cnv_ctx* c, const T* src, T* b0, T* b1, int negate) {
-    // FP32 planes (the defect-correction inner solve) may relax with their own factor
-    const double beta = (sizeof(T) == 4 && c->p.inner_beta > 0) ? c->p.inner_beta : c->p.beta;
-    return sor_args_for<T>(c->n, c->ld, c->p.dx, c->p.dy, beta, c->tiles_i, c->tiles_j, c->partial, c->ctl, src, b0,
-                           b1, negate);
+// The context's FP64 solve: source src, planes psi[0..2]; f = -src when negate.
+inline cnv::SorArgs<double> sor_args64(cnv_ctx* c, const double* src, int negate) {
+    return sor_args_for<double>(c->n, c->ld, c->p.dx, c->p.dy, c->p.beta, c->tiles_i, c->tiles_j, c->partial, c->ctl,
+                                src, c->psi[0], c->psi[1], c->psi[2], negate);
+}
+// The context's FP32 inner solve of defect correction: source r32, planes e32[0..1]; the inner
+// solves may relax with their own factor.
+inline cnv::SorArgs<float> sor_args32(cnv_ctx* c) {
+    const double beta = c->p.inner_beta > 0 ? c->p.inner_beta : c->p.beta;
+    return sor_args_for<float>(c->n, c->ld, c->p.dx, c->p.dy, beta, c->tiles_i, c->tiles_j, c->partial, c->ctl,
+                               c->r32, c->e32[0], c->e32[1], nullptr, 0);
+}
+// The planes of the context's solve for the kernels that read its final iterate: FP64 the three
+// SOR planes; the refined form's final iterate is always the FP64 accumulator psi[0].
+inline cnv::Planes planes_of(const cnv_ctx* c) {
+    if (c->p.precision == CNV_PREC_FP32_DC) return cnv::Planes{{c->psi[0], c->psi[0], c->psi[0]}};
+    return cnv::Planes{{c->psi[0], c->psi[1], c->psi[2]}};
 }
 
 // The default SOR launch shape of an n x n grid on a device with `cus` compute units.
@@ -148,7 +162,8 @@ inline int launch_sor(cnv_ctx* c, cnv::SorArgs<float> a, int L, bool gs) {
     return cnv::sor_fp32_launch(a, c->shape, gs, c->stream);
 }
 
-inline bool sor_done(int status) { return status != cnv::SOR_RUNNING; }
+// A solve has ended (ROLLBACK is not an end: the next launch still writes the final iterate).
+inline bool sor_done(int status) { return status != cnv::SOR_RUNNING && status != cnv::SOR_ROLLBACK; }
 
 // A finished solve's slot status as a step status of cnv.h.
 inline int solve_status(int sor_status) {

@@ -43,22 +43,21 @@ __global__ void __launch_bounds__(kBlock) poisson_init_kernel(double* psi0, SorC
 
 // warm start (opt-in): the previous solve's final iterate, psi[fin->parity], becomes psi0, the
 // first iterate of the next solve (read before the slots are reset by the init kernel)
-__global__ void __launch_bounds__(kBlock) warm_start_kernel(double* psi0, const double* psi1,
-                                                           const SorCtl* fin, long long count) {
-    if (!fin->parity) return;  // already in psi0
+__global__ void __launch_bounds__(kBlock) warm_start_kernel(double* psi0, Planes psi, const SorCtl* fin,
+                                                           long long count) {
+    const double* src = psi.p[fin->parity];
+    if (src == psi0) return;  // already in psi0
     const long long stride = (long long)gridDim.x * kBlock;
-    for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) psi0[q] = psi1[q];
+    for (long long q = (long long)blockIdx.x * kBlock + threadIdx.x; q < count; q += stride) psi0[q] = src[q];
 }
 
 // ---- velocities u = DY psi, v = -(DX psi) at every point (src/main.c:366-383) --------------
-__global__ void __launch_bounds__(kBlock) velocity_kernel(const double* psi0, const double* psi1,
-                                                         const SorCtl* fin, double* __restrict__ u,
-                                                         double* __restrict__ v, int n, int ld,
-                                                         Stencils s) {
+__global__ void __launch_bounds__(kBlock) velocity_kernel(Planes psi, const SorCtl* fin, double* __restrict__ u,
+                                                         double* __restrict__ v, int n, int ld, Stencils s) {
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (i >= n || j >= n) return;
-    velocity_at(fin->parity ? psi1 : psi0, u, v, ld, s, i, j);
+    velocity_at(psi.p[fin->parity], u, v, ld, s, i, j);
 }
 
 // ---- continuity diagnostic: max/min of DX u + DY v (src/main.c:387-408, fluiddyn.c:149,
@@ -106,12 +105,11 @@ __global__ void __launch_bounds__(kBlock) copy_kernel(double* dst, const double*
 // State snapshot for output: dst = [w | psi | u | v], each `count` doubles, with psi read from the
 // buffer that holds the solve's final iterate (fin->parity), or from psi0 when psi_fixed (the
 // refined form accumulates psi there).
-__global__ void __launch_bounds__(kBlock) snapshot_kernel(const double* __restrict__ w, const double* psi0,
-                                                         const double* psi1, const double* __restrict__ u,
+__global__ void __launch_bounds__(kBlock) snapshot_kernel(const double* __restrict__ w, Planes planes,
+                                                         const double* __restrict__ u,
                                                          const double* __restrict__ v, const SorCtl* fin,
-                                                         int psi_fixed, double* __restrict__ dst,
-                                                         long long count) {
-    const double* __restrict__ psi = (psi_fixed || !(fin->parity & 1)) ? psi0 : psi1;
+                                                         double* __restrict__ dst, long long count) {
+    const double* __restrict__ psi = planes.p[fin->parity];
     const long long q = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (q < count) {
         dst[q] = w[q];
@@ -159,18 +157,18 @@ int poisson_init_launch(double* psi0, SorCtl* ctl, Diag* diag, int n, int ld, do
     return launched();
 }
 
-int warm_start_launch(double* psi0, const double* psi1, const SorCtl* fin, int n, int ld, hipStream_t st) {
+int warm_start_launch(double* psi0, const Planes& psi, const SorCtl* fin, int n, int ld, hipStream_t st) {
     const long long count = (long long)n * ld;
     unsigned nb = blocks_for(count);
     if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(warm_start_kernel, dim3(nb), dim3(kBlock), 0, st, psi0, psi1, fin, count);
+    hipLaunchKernelGGL(warm_start_kernel, dim3(nb), dim3(kBlock), 0, st, psi0, psi, fin, count);
     return launched();
 }
 
-int velocity_launch(const double* psi0, const double* psi1, const SorCtl* fin, double* u, double* v, int n,
-                    int ld, const Stencils& s, hipStream_t st) {
+int velocity_launch(const Planes& psi, const SorCtl* fin, double* u, double* v, int n, int ld, const Stencils& s,
+                    hipStream_t st) {
     dim3 grid((n + 63) / 64, (n + 3) / 4);
-    hipLaunchKernelGGL(velocity_kernel, grid, dim3(kBlock), 0, st, psi0, psi1, fin, u, v, n, ld, s);
+    hipLaunchKernelGGL(velocity_kernel, grid, dim3(kBlock), 0, st, psi, fin, u, v, n, ld, s);
     return launched();
 }
 
@@ -199,10 +197,10 @@ int copy_launch(double* dst, const double* src, long long count, hipStream_t st)
     return launched();
 }
 
-int snapshot_launch(const double* w, const double* psi0, const double* psi1, const double* u, const double* v,
-                    const SorCtl* fin, int psi_fixed, double* dst, long long count, hipStream_t st) {
-    hipLaunchKernelGGL(snapshot_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, st, w, psi0, psi1, u, v, fin,
-                       psi_fixed, dst, count);
+int snapshot_launch(const double* w, const Planes& psi, const double* u, const double* v, const SorCtl* fin,
+                    double* dst, long long count, hipStream_t st) {
+    hipLaunchKernelGGL(snapshot_kernel, dim3(blocks_for(count)), dim3(kBlock), 0, st, w, psi, u, v, fin, dst,
+                       count);
     return launched();
 }
 

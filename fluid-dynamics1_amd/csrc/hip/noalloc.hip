@@ -1,8 +1,8 @@
 // noalloc.hip -- the Poisson solve of cnv_poisson_device without a context and without any
 // device allocation (include/cnv.h: cnv_poisson_device_noalloc).
 //
-// The solve's scratch -- the second SOR iterate of the ping-pong pair, the per-tile L1 partial
-// sums and the two state slots -- lives in `scratch`, a file-scope device array of the library
+// The solve's scratch -- the second and third SOR iterate planes, the per-tile L1 partial sums and
+// the two state slots -- lives in `scratch`, a file-scope device array of the library
 // sized for grids up to kNoAllocMaxN per side; each GPU the library runs on holds its own copy
 // until the process ends.  Host code never passes the array's name to a kernel (on the host it
 // is only a stand-in for the symbol): the first call on each device, with that device current,
@@ -10,8 +10,8 @@
 // every kernel argument and copy uses that address.  Every call on one GPU shares the array, so
 // calls that overlap on one GPU race; each call resets what must start at zero (the first
 // iterate and the state slots, by the Poisson init kernel) and needs nothing else from the last
-// call's contents: launch 0 writes every point of the second iterate before any launch reads it,
-// and a launch reads only partial sums that the launch before it wrote.
+// call's contents: launch L writes every point of plane (L+1) % 3 before any launch reads it, and
+// a launch's decider reads only partial sums that the launch before it wrote.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -25,7 +25,8 @@ using namespace cnv;
 namespace {
 
 constexpr int kNoAllocMaxN = CNV_NOALLOC_MAX_N;
-constexpr size_t kIterDoubles = (size_t)kNoAllocMaxN * kNoAllocMaxN;
+constexpr size_t kPlaneDoubles = (size_t)kNoAllocMaxN * kNoAllocMaxN;
+constexpr size_t kIterDoubles = 2 * kPlaneDoubles;  // planes 1 and 2 (plane 0 is the caller's psi)
 // tiles of the thinnest compiled default tile (4 x 108 owned points) at the largest side
 constexpr size_t kMaxTiles = 2560;
 constexpr size_t kPartialDoubles = (size_t)2 * 2 * kMaxSweepsPerLaunch * kMaxTiles;
@@ -87,14 +88,15 @@ int cnv_poisson_device_noalloc(int n, double dx, double dy, double beta, double 
     int rc = scratch_of(dev, &base);
     if (rc) return rc;
     double* psi1 = base;
+    double* psi2 = base + kPlaneDoubles;
     double* partial = base + kIterDoubles;
     SorCtl* ctl = reinterpret_cast<SorCtl*>(base + kIterDoubles + kPartialDoubles);
     hipStream_t st = nullptr;  // the null stream
     if (poisson_init_launch(d_psi, ctl, nullptr, n, n, tol, itmax, st))
         return cnv_fail(nullptr, CNV_ERR_HIP, "Poisson init launch failed: %s", hipGetErrorString(hipGetLastError()));
-    SorArgs<double> a = sor_args_for<double>(n, n, dx, dy, beta, ti, tj, partial, ctl, d_f, d_psi, psi1, 0);
+    SorArgs<double> a = sor_args_for<double>(n, n, dx, dy, beta, ti, tj, partial, ctl, d_f, d_psi, psi1, psi2, 0);
     const bool gs = poisson_type == 1;
-    const long long cap = (long long)itmax / sh.s + 4LL * kLaunchesPerPoll + 8;  // working launches <= itmax/S + 2
+    const long long cap = (long long)itmax / sh.s + 4LL * kLaunchesPerPoll + 8;  // working launches <= itmax/S + 3
     long long queued = 0;
     SorCtl fin{};
     while (true) {
@@ -112,8 +114,9 @@ int cnv_poisson_device_noalloc(int n, double dx, double dy, double beta, double 
         if (sor_done(fin.status)) break;
         if (queued > cap) return cnv_fail(nullptr, CNV_ERR_HIP, "SOR state did not terminate after %lld launches", queued);
     }
-    if (fin.parity & 1) {  // the final iterate is in the scratch half of the pair
-        if (hipMemcpyAsync(d_psi, psi1, sizeof(double) * n * n, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+    if (fin.parity != 0) {  // the final iterate is in a scratch plane
+        if (hipMemcpyAsync(d_psi, fin.parity == 1 ? psi1 : psi2, sizeof(double) * n * n, hipMemcpyDeviceToDevice,
+                           st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             return cnv_fail(nullptr, CNV_ERR_HIP, "copying the final iterate failed: %s",
                             hipGetErrorString(hipGetLastError()));

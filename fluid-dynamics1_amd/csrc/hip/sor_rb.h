@@ -177,10 +177,23 @@ __device__ inline double wave_sum(double x) {
     return v[0];
 }
 
-// One workgroup's launch on tile `tile` of the solve described by `a` (NW*64 threads).
+// Plane idx of a solve's iterate planes (FP64: 0, 1, 2; FP32: 0, 1 -- psi2 is not used).
+template <typename T>
+__device__ inline T* sor_plane(const SorArgs<T>& a, int idx) {
+    if constexpr (SorPlanes<T>::value == 2) return (idx & 1) ? a.psi1 : a.psi0;
+    else return idx == 0 ? a.psi0 : (idx == 1 ? a.psi1 : a.psi2);
+}
+
+// One sweeping workgroup's launch on tile `tile` of the solve described by `a` (NW*64 threads).
+// It reads the state slot the previous launch's decider wrote: RUNNING -- sweep on from plane
+// L % NB (the previous launch's output) into plane (L+1) % NB, as many sweeps as the budget
+// leaves, and leave this tile's per-sweep L1 sums for the next launch's decider; ROLLBACK -- the
+// solve converged inside the launch before last: redo rb_n sweeps from plane rb_src into plane
+// `parity`, which then holds the final iterate; anything else -- the solve has ended: return.
 template <typename T, int S, int NW, int RB, bool GS, bool XR, bool NORM>
-__device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile) {
+__device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int tile) {
 #pragma clang fp contract(off)
+    constexpr int NB = SorPlanes<T>::value;
     constexpr int H = 2 * S;
     // the iterate's L1 norm is needed by the relative stopping rule (rtol): always on FP32
     // planes, on request (NORM) for FP64.  It is measured once per launch, after its last sweep.
@@ -204,12 +217,9 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     // and the per-lane iterate norm after the last sweep
     __shared__ double lred[SE][NW * 64];
     __shared__ double lnrm[kNorm ? NW * 64 : 1];
-    constexpr int nq = SE + (kNorm ? 1 : 0);  // reduced quantities: SE updates (and the norm)
-    __shared__ double lw[NW][nq];       // per-wave totals of the previous launch's partials
 
     const int L = a.launch;
     const SorCtl* prev = a.ctl + ((L + 1) & 1);
-    SorCtl* mine = a.ctl + (L & 1);
     const int tid = threadIdx.x;
     const int ntiles = a.tiles_i * a.tiles_j;
     const int ti = tile / a.tiles_j;
@@ -223,9 +233,9 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
     const int row0 = wv * RB;        // even: the colour of (row0 + r, 2m + q) is (r + q) & 1
 
     // Speculative tile load for the common case (sweep on from the previous launch's output),
-    // issued before the state slot and the partials are read so that all three are in flight
-    // together.  Column facts: q = 0 -> column 2m, q = 1 -> column 2m+1.
-    const T* __restrict__ srcN = (L & 1) ? a.psi1 : a.psi0;
+    // issued before the state slot is read so that both are in flight together.  Column facts:
+    // q = 0 -> column 2m, q = 1 -> column 2m+1.
+    const T* __restrict__ srcN = sor_plane(a, L % NB);
     bool colin[2], ownc[2], colv[2];
     int gjq[2];
 #pragma unroll
@@ -276,69 +286,33 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         }
     }
 
-    // the previous launch's per-tile partials: thread t sums tiles t, t + NT, ... in that order
-    // (all S slots are read, before the state slot is known, so that these loads are in flight
-    // together with the tile and the slot; only the first sd are used).  Slot layout per tile:
-    // [s][tile][0] the L1 update of sweep s (FP32: [0][tile][0], the launch's last sweep),
-    // [0][tile][1] the iterate's norm after the last sweep.
-    const size_t pstride = (size_t)kMaxSweepsPerLaunch * ntiles * 2;
-    double pv[nq];  // pv[s]: updates; pv[S]: norm
+    // what this launch does: the state after launch L-2 and the sweeps launch L-1 performed
+    const int pstatus = prev->status;
+    bool rollback = false;
+    int s_run = 0;
+    T* __restrict__ dst = nullptr;
+    if (pstatus == SOR_RUNNING) {
+        const int budget = prev->itmax - (prev->k + prev->s_done);
+        s_run = budget < S ? budget : S;
+        dst = sor_plane(a, (L + 1) % NB);
+    } else if (pstatus == SOR_ROLLBACK) {
+        rollback = true;
+        s_run = prev->rb_n;
+        dst = sor_plane(a, prev->parity);
+    }
+    if (s_run <= 0) return;  // the solve has ended, or its budget is spent
+    if (rollback) {
+        // the converged launch's input replaces the speculatively loaded tile
+        const T* __restrict__ srcR = sor_plane(a, prev->rb_src);
 #pragma unroll
-    for (int i = 0; i < nq; ++i) pv[i] = 0.0;
-    {
-        const double* pp = a.partial + ((L + 1) & 1) * pstride;
-        constexpr int NT = NW * 64;
-        if (ntiles <= NT) {  // one tile per thread at most (1024^2: 240 tiles)
-            if (tid < ntiles) {
-#pragma unroll
-                for (int s = 0; s < SE; ++s) pv[s] = pp[((size_t)s * ntiles + tid) * 2];
-                if (kNorm) pv[nq - 1] = pp[(size_t)tid * 2 + 1];
-            }
-        } else {
-            // several tiles per thread: U at a time, all loads issued before the sums, so that
-            // the round trips overlap instead of following one another (same order): FP64 launch
-            // at 4096^2 244 -> 214 us with U = 4; U = 2 where four tiles' values beside the loaded
-            // tile would exceed the register budget.  The FP32 planes measured 1-4% slower with
-            // U = 2 or 4 and keep U = 1.
-            constexpr int U = sizeof(T) == 4 ? 1 : (nq > 6 ? 2 : 4);
-            for (int t0 = tid; t0 < ntiles; t0 += U * NT) {
-                double x[U][nq];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int t = t0 + u * NT;
-                    const int tc = t < ntiles ? t : 0;  // in range; its value is discarded
-#pragma unroll
-                    for (int s = 0; s < SE; ++s) x[u][s] = pp[((size_t)s * ntiles + tc) * 2];
-                    if (kNorm) x[u][nq - 1] = pp[(size_t)tc * 2 + 1];
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if (t0 + u * NT < ntiles) {
-#pragma unroll
-                        for (int i = 0; i < nq; ++i) pv[i] += x[u][i];
-                    }
-                }
-            }
+        for (int r = 0; r < RB; ++r) {
+            const int gi = gi0 + row0 + r;
+            const bool rin = (unsigned)gi < (unsigned)nr;
+            load_pair(srcR, (long long)(rin ? gi : 0) * ld + gjq[0], rin && colv[0], rin && colv[0],
+                      rin && colv[1], vec, v[0][r], v[1][r]);
         }
     }
-    const int pstatus = prev->status;
-    if (pstatus != SOR_RUNNING) {  // the solve already ended: carry its state forward
-        if (tile == 0 && tid == 0) *mine = *prev;
-        return;
-    }
-    const int sd = prev->s_done;   // sweeps the previous launch performed (0: first launch)
-    const int k_prev = prev->k;
-    const int itmax = prev->itmax;
-    const double tol = prev->tol, rtol = prev->rtol;
 
-    // Sweep first, decide afterwards.  Unless the previous launch ended the solve, this launch
-    // performs s_spec = min(S, itmax - (k_prev + sd)) sweeps from the previous output, which the
-    // slot alone determines; the partials the decision needs have long arrived by the time the
-    // sweeps end.  If the decision then ends the solve, the sweeps are discarded; if the solve
-    // converged inside the previous launch (rollback), the tile is reloaded from that launch's
-    // untouched input buffer and swept again (second pass).
-    const int budget = itmax - (k_prev + sd);
-    const int s_spec = budget < S ? (budget > 0 ? budget : 0) : S;
     const T c1 = a.c1, c2 = a.c2, B = a.B, rB = a.rB, omb = a.omb;
     // half-sweep 0 updates colour 0 and reads its neighbours' colour-1 boundary rows: colour 1
     // sits in column q = (r + 1) & 1 of row r
@@ -399,46 +373,39 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         if (kNorm && with_norm)
             lnrm[tid] = (double)((ownc[0] ? nrm[0] : Acc(0)) + (ownc[1] ? nrm[1] : Acc(0)));
     };
-    // a pass of ns sweeps; the rollback pass (rare) and a budget-limited last launch take the
-    // loop, which sums the FP64 norm in every sweep (the last one's stays)
-    auto sweeps = [&](const int ns) {
-        sweeps_begin();
-        for (int s = 0; s < ns; ++s) one_sweep(s, kLastOnly ? s == ns - 1 : kNorm);
-    };
-    int mode = 0;          // 0: sweep on, 1: stop (terminal), 2: redo s_run sweeps (rollback)
-    int s_run = s_spec;
     // the common full pass, unrolled: the values stay in their registers from sweep to sweep (a
     // rolled loop makes the compiler rotate them through copies at every iteration); not for the
-    // FP64 shapes whose unrolled pass no longer fits the register budget of their wave count
+    // FP64 shapes whose unrolled pass no longer fits the register budget of their wave count.
+    // A rollback or a budget-limited last launch takes the loop, which sums the FP64 norm in
+    // every sweep (the last one's stays).
     constexpr bool kUnroll = sizeof(T) == 4 || (S * RB <= 40 && NW != 12 && !(kNorm && S * RB > 20));
-    if (kUnroll && s_spec == S) {
-        sweeps_begin();
+    sweeps_begin();
+    if (kUnroll && s_run == S) {
 #pragma unroll
         for (int s = 0; s < S; ++s) one_sweep(s, s == S - 1);
     } else {
-        sweeps(s_spec);
+        for (int s = 0; s < s_run; ++s) one_sweep(s, kLastOnly ? s == s_run - 1 : kNorm);
     }
 
-    // ---- reductions: one barrier for both --------------------------------------------------
-    // The previous launch's partials: the waves whose threads hold some (thread t summed tiles
-    // t, t + NT, ...) reduce them per (sweep, quantity) across their lanes; the per-wave totals
-    // go to LDS.  A barrier waits for outstanding global stores too, so none is issued before it.
-    // All S slots are reduced (only the first sd are read): no branch separates the chains.
-    const int nwp = min(NW, (ntiles + 63) >> 6);  // waves holding partials
-    if (sd > 0 && wv < nwp) {
-        wave_sum_n(pv);
-        if (lane == 0) {
+    // the owned tile to its plane; domain-boundary points hold +-0: store +0
 #pragma unroll
-            for (int i = 0; i < nq; ++i) lw[wv][i] = pv[i];
+    for (int r = 0; r < RB; ++r) {
+        const int li = row0 + r;
+        const int gi = gi0 + li;
+        if (li >= H && li < H + TI && gi < nr) {  // owned row (wave-uniform)
+            const bool rint = gi >= 1 && gi <= nr - 2;
+            store_pair(dst, (long long)gi * ld + gjq[0], ownc[0], ownc[1], vec,
+                       (rint && colin[0]) ? v[0][r] : T(0), (rint && colin[1]) ? v[1][r] : T(0));
         }
     }
-    __syncthreads();  // this launch's lane sums (lred) and the per-wave totals (lw) are complete
+    if (rollback) return;  // the final iterate: no decision follows
 
-    // this launch's per-tile partials (the sweeps' per-lane sums in lred, the norm in lnrm):
-    // wave w reduces rows, in a fixed order.  Written whatever the decision below: a launch that
-    // ends the solve leaves partials nobody reads.
-    const int nupd = kLastOnly ? 1 : s_spec;  // update rows (FP32: the last sweep's)
-    const int nrows = s_spec > 0 ? nupd + (kNorm ? 1 : 0) : 0;
+    // this tile's per-sweep partials (the per-lane sums in lred, the norm in lnrm) for the next
+    // launch's decider: wave w reduces rows w, w + NW, ..., each in a fixed order
+    __syncthreads();  // every wave's lane sums are in LDS
+    const size_t pstride = (size_t)kMaxSweepsPerLaunch * ntiles * 2;
+    const int nupd = kLastOnly ? 1 : s_run;  // update rows (FP32: the last sweep's)
+    const int nrows = nupd + (kNorm ? 1 : 0);
     for (int row = wv; row < nrows; row += NW) {
         const double* col = row < nupd ? lred[row] : lnrm;
         double acc = 0.0;
@@ -448,12 +415,100 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         const size_t slot = row < nupd ? ((size_t)row * ntiles + tile) * 2 : (size_t)tile * 2 + 1;
         if (lane == 0) a.partial[(L & 1) * pstride + slot] = acc;
     }
+}
 
-    // ---- decision on the previous launch (src/poisson.c:272-278): lane i of every wave sums the
-    //      per-wave totals of quantity i in wave order, the same in every workgroup, so all reach
-    //      the same decision; the totals are then broadcast to the whole wave -------------------
-    double etot[SE];     // the previous launch's L1 update of sweep s (FP32: of its last sweep)
-    double ntot = 0.0;   // and its iterate's L1 norm after its last sweep (relative rule)
+// The deciding workgroup of launch L (NW*64 threads): decides on launch L-1's sweeps from their
+// per-tile partial sums (src/poisson.c:272-278) and writes state slot L&1, which launch L+1's
+// workgroups and the host read.  Every sum runs in one fixed order: thread t adds the partials of
+// tiles t, t + NT, ... in that order, each wave reduces its threads' sums with DPP moves, and lane
+// i adds the per-wave totals of quantity i in wave order.
+template <typename T, int S, int NW, bool NORM>
+__device__ __forceinline__ void sor_rb_decider(const SorArgs<T>& a) {
+    constexpr int NB = SorPlanes<T>::value;
+    constexpr bool kNorm = sizeof(T) == 4 || NORM;
+    constexpr bool kLastOnly = sizeof(T) == 4;
+    constexpr int SE = kLastOnly ? 1 : S;
+    constexpr int nq = SE + (kNorm ? 1 : 0);  // reduced quantities: SE updates (and the norm)
+    __shared__ double lw[NW][nq];             // per-wave totals
+
+    const int L = a.launch;
+    const SorCtl* prev = a.ctl + ((L + 1) & 1);
+    SorCtl* mine = a.ctl + (L & 1);
+    const int tid = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ntiles = a.tiles_i * a.tiles_j;
+
+    // the previous launch's per-tile partials (all S rows are read, before the slot is known, so
+    // that these loads are in flight together with it; only the first sd are used).  Layout per
+    // tile: [s][tile][0] the L1 update of sweep s (FP32: [0][tile][0], the launch's last sweep),
+    // [0][tile][1] the iterate's norm after the last sweep.
+    const size_t pstride = (size_t)kMaxSweepsPerLaunch * ntiles * 2;
+    double pv[nq];
+#pragma unroll
+    for (int i = 0; i < nq; ++i) pv[i] = 0.0;
+    {
+        const double* pp = a.partial + ((L + 1) & 1) * pstride;
+        constexpr int NT = NW * 64;
+        if (ntiles <= NT) {  // one tile per thread at most (1024^2: 240 tiles)
+            if (tid < ntiles) {
+#pragma unroll
+                for (int s = 0; s < SE; ++s) pv[s] = pp[((size_t)s * ntiles + tid) * 2];
+                if (kNorm) pv[nq - 1] = pp[(size_t)tid * 2 + 1];
+            }
+        } else {
+            // several tiles per thread: U at a time, all loads issued before the sums, so that
+            // the round trips overlap instead of following one another (same order)
+            constexpr int U = 4;
+            for (int t0 = tid; t0 < ntiles; t0 += U * NT) {
+                double x[U][nq];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = t0 + u * NT;
+                    const int tc = t < ntiles ? t : 0;  // in range; its value is discarded
+#pragma unroll
+                    for (int s = 0; s < SE; ++s) x[u][s] = pp[((size_t)s * ntiles + tc) * 2];
+                    if (kNorm) x[u][nq - 1] = pp[(size_t)tc * 2 + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (t0 + u * NT < ntiles) {
+#pragma unroll
+                        for (int i = 0; i < nq; ++i) pv[i] += x[u][i];
+                    }
+                }
+            }
+        }
+    }
+    const SorCtl P = *prev;
+    if (P.status != SOR_RUNNING) {
+        // ended: carry the state forward; a rollback is being done by this launch's sweepers,
+        // after which plane P.parity holds the final iterate
+        if (tid == 0) {
+            SorCtl next = P;
+            if (P.status == SOR_ROLLBACK) {
+                next.status = SOR_CONVERGED;
+                next.launches += 1;
+            }
+            next.s_done = 0;
+            *mine = next;
+        }
+        return;
+    }
+    const int sd = P.s_done;  // sweeps the previous launch performed (0: none, e.g. launch 0)
+    const int nwp = min(NW, (ntiles + 63) >> 6);  // waves holding partials
+    if (sd > 0 && wv < nwp) {
+        wave_sum_n(pv);
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < nq; ++i) lw[wv][i] = pv[i];
+        }
+    }
+    __syncthreads();
+    if (wv != 0) return;
+    // lane i of wave 0 sums the per-wave totals of quantity i in wave order; broadcast
+    double etot[SE];
+    double ntot = 0.0;
     {
         double e = 0.0;
         if (sd > 0 && (lane < (kLastOnly ? 1 : sd) || (kNorm && lane == nq - 1))) {
@@ -466,16 +521,11 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         if (kNorm)
             ntot = __hiloint2double(__builtin_amdgcn_readlane(hi, nq - 1), __builtin_amdgcn_readlane(lo, nq - 1));
     }
-    SorCtl next;
-    next.k = k_prev;
-    next.parity = L & 1;   // where the previous launch left its output
+    const int k_prev = P.k, itmax = P.itmax;
+    const double tol = P.tol, rtol = P.rtol;
+    SorCtl next = P;
+    next.parity = L % NB;  // where the previous launch left its output
     next.status = SOR_RUNNING;
-    next.itmax = itmax;
-    next.tol = tol;
-    next.rtol = rtol;
-    next.launches = prev->launches;
-    next.e_last = prev->e_last;
-    next.n_last = prev->n_last;
     if (sd > 0) {
         // the first sweep s < sd with e < tol (converged) or, failing that, below rtol times the
         // iterate's norm after the launch (fold); static indices keep etot in registers.  Sum i
@@ -496,83 +546,46 @@ __device__ __forceinline__ void sor_rb_body(const SorArgs<T>& a, const int tile)
         if (kNorm && s_conv < 0 && s_fold < 0 && s_bad < 0 && !(ntot <= DBL_MAX)) s_bad = sd - 1;
         const int s_rep = (s_conv >= 0 && s_conv < sd - 1) ? s_conv : sd - 1;  // reported sweep
         double e_rep = 0.0;
-        const double n_rep = ntot;
 #pragma unroll
         for (int i = 0; i < SE; ++i) {
             if ((kLastOnly ? sd - 1 : i) == s_rep) e_rep = etot[i];
         }
+        next.e_last = e_rep;
+        next.n_last = ntot;
         if (s_conv >= 0 && s_conv < sd - 1) {
-            // converged inside the previous launch: redo s_conv+1 sweeps from its
-            // (untouched) input buffer; that iterate is the final one (src/poisson.c:273)
-            mode = 2;
-            s_run = s_conv + 1;
-            next.status = SOR_CONVERGED;
-            next.k = k_prev + s_run;
-            next.e_last = e_rep;
-            next.n_last = n_rep;
+            // converged inside the previous launch: the next launch redoes s_conv+1 sweeps from
+            // that launch's input (plane (L-1) % NB, untouched since), into plane L % NB (its
+            // output, no longer needed); that iterate is the final one (src/poisson.c:273)
+            next.status = SOR_ROLLBACK;
+            next.k = k_prev + s_conv + 1;
+            next.rb_src = (L + NB - 1) % NB;
+            next.rb_n = s_conv + 1;
         } else {
             next.k = k_prev + sd;
-            next.e_last = e_rep;
-            next.n_last = n_rep;
-            if (s_conv >= 0) {
-                next.status = SOR_CONVERGED;
-                mode = 1;
-            } else if (s_fold >= 0) {
-                next.status = SOR_FOLD;
-                mode = 1;
-            } else if (s_bad >= 0) {
-                next.status = SOR_DIVERGED;
-                mode = 1;
-            } else if (next.k >= itmax) {
-                next.status = SOR_FAILED;
-                mode = 1;
-            }
+            if (s_conv >= 0) next.status = SOR_CONVERGED;
+            else if (s_fold >= 0) next.status = SOR_FOLD;
+            else if (s_bad >= 0) next.status = SOR_DIVERGED;
+            else if (next.k >= itmax) next.status = SOR_FAILED;
         }
     } else if (itmax <= k_prev) {
         next.status = SOR_FAILED;
-        mode = 1;
     }
-    next.s_done = (mode == 0) ? s_run : 0;  // mode 0: s_run == s_spec, the sweeps just done
-    if (mode != 1) next.launches += 1;
-    if (tile == 0 && tid == 0) *mine = next;
-    if (mode == 2) {
-        // rollback: the previous launch's input, swept s_run times, replaces its output
-        const T* __restrict__ srcR = (L & 1) ? a.psi0 : a.psi1;
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-            const int gi = gi0 + row0 + r;
-            const bool rin = (unsigned)gi < (unsigned)nr;
-            load_pair(srcR, (long long)(rin ? gi : 0) * ld + gjq[0], rin && colv[0], rin && colv[0],
-                      rin && colv[1], vec, v[0][r], v[1][r]);
-        }
-        __syncthreads();  // every wave is done reading xch of the first pass
-        sweeps(s_run);
-    }
-    if (mode == 1) return;
-    // normal: the previous launch's output was read, the other buffer is written; rollback: the
-    // previous launch's input was read, its output is overwritten
-    T* __restrict__ dst = ((L & 1) ^ (mode == 2 ? 0 : 1)) ? a.psi1 : a.psi0;
-
-#pragma unroll
-    for (int r = 0; r < RB; ++r) {
-        const int li = row0 + r;
-        const int gi = gi0 + li;
-        if (li >= H && li < H + TI && gi < nr) {  // owned row (wave-uniform)
-            const bool rint = gi >= 1 && gi <= nr - 2;
-            // domain-boundary points hold +-0: store +0
-            store_pair(dst, (long long)gi * ld + gjq[0], ownc[0], ownc[1], vec,
-                       (rint && colin[0]) ? v[0][r] : T(0), (rint && colin[1]) ? v[1][r] : T(0));
-        }
-    }
+    // the sweeps this launch's sweepers perform (they computed the same from the slot they read)
+    const int budget = itmax - (k_prev + sd);
+    next.s_done = next.status == SOR_RUNNING ? (budget < S ? (budget > 0 ? budget : 0) : S) : 0;
+    if (next.status == SOR_RUNNING && next.s_done > 0) next.launches = P.launches + 1;
+    if (lane == 0) *mine = next;
 }
 
 template <typename T, int S, int NW, int RB, bool GS, bool XR, bool NORM = false>
 __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
-    sor_rb_body<T, S, NW, RB, GS, XR, NORM>(a, sor_tile_of_block(blockIdx.x, a.tiles_i * a.tiles_j));
+    const int ntiles = a.tiles_i * a.tiles_j;
+    if ((int)blockIdx.x == ntiles) sor_rb_decider<T, S, NW, NORM>(a);
+    else sor_rb_sweeper<T, S, NW, RB, GS, XR, NORM>(a, sor_tile_of_block(blockIdx.x, ntiles));
 }
 
-// Batched form: workgroup b runs tile map[b].tile of solve map[b].inst (inst < 0: padding).
-// All solves of a batch share the launch index L.
+// Batched form: workgroup b runs tile map[b].tile of solve map[b].inst (inst < 0: padding; tile
+// == the solve's tile count: its decider).  All solves of a batch share the launch index L.
 template <typename T, int S, int NW, int RB, bool GS, bool XR>
 __global__ void __launch_bounds__(NW * 64) sor_rb_batch_kernel(const SorArgs<T>* __restrict__ args,
                                                                const SorBlock* __restrict__ map, int L) {
@@ -580,13 +593,14 @@ __global__ void __launch_bounds__(NW * 64) sor_rb_batch_kernel(const SorArgs<T>*
     if (m.inst < 0) return;
     SorArgs<T> a = args[m.inst];
     a.launch = L;
-    sor_rb_body<T, S, NW, RB, GS, XR, false>(a, m.tile);
+    if (m.tile == a.tiles_i * a.tiles_j) sor_rb_decider<T, S, NW, false>(a);
+    else sor_rb_sweeper<T, S, NW, RB, GS, XR, false>(a, m.tile);
 }
 
 template <typename T, int S, int NW, int RB>
 inline int sor_launch_shape(const SorArgs<T>& a, bool gs, hipStream_t st) {
     const int ntiles = a.tiles_i * a.tiles_j;
-    const dim3 grid(ntiles), block(NW * 64);
+    const dim3 grid(ntiles + 1), block(NW * 64);  // the tiles' sweepers and one decider
     // FP32 always multiplies by RN(1/B); FP64 does when 1/B is exact
     const bool xr = sizeof(T) == 4 || a.exact_rcp;
     if constexpr (sizeof(T) == 8) {
@@ -621,6 +635,7 @@ inline int sor_launch_shape(const SorArgs<T>& a, bool gs, hipStream_t st) {
     X(6, 16, 4)           \
     X(5, 12, 6)           \
     X(5, 8, 4)            \
+    X(5, 8, 8)            \
     X(4, 8, 4)            \
     X(5, 4, 6)
 

@@ -116,6 +116,8 @@ __device__ inline void ctl_reset(SorCtl* sl, double tol, double rtol, int itmax)
     sl->s_done = 0;
     sl->launches = 0;
     sl->itmax = itmax;
+    sl->rb_src = 0;
+    sl->rb_n = 0;
     sl->e_last = 0.0;
     sl->tol = tol;
     sl->rtol = rtol;

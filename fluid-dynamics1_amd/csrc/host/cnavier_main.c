@@ -67,6 +67,28 @@ static mtrx to_mtrx(const double *a, int n)
 }
 
 /* The four VTK files of one output step, in the reference's order (src/main.c:429-436). */
+/* The centreline profiles of step `step`: u(x = 0.5, y) is column c = n/2 of u and v(x, y = 0.5)
+ * row c of v (node j sits at x = j*dx, src/main.c:138-139, so x = 0.5 is node n/2 for even n;
+ * the nearest node below it for odd n), one line per node k with its coordinate k*h, full
+ * precision. */
+static void write_centerlines(const double *u, const double *v, int n, double h, int step, const char *dir)
+{
+    char name[512];
+    snprintf(name, sizeof(name), "%s/centerlines-%d.csv", dir, step);
+    FILE *f = fopen(name, "w");
+    if (!f) {
+        printf("\nError while opening file: %s\n", name);
+        return;
+    }
+    const int c = n / 2;
+    fprintf(f, "# step %d, n %d: u(x=%.17g, y) at column %d, v(x, y=%.17g) at row %d\n", step, n, c * h, c,
+            c * h, c);
+    fprintf(f, "k,coord,u_center,v_center\n");
+    for (int k = 0; k < n; k++)
+        fprintf(f, "%d,%.17g,%.17g,%.17g\n", k, k * h, u[(size_t)k * n + c], v[(size_t)c * n + k]);
+    fclose(f);
+}
+
 static void write_vtk_set(const double *w, const double *psi, const double *u, const double *v, int n,
                           const char *dir)
 {
@@ -88,6 +110,9 @@ typedef struct {
     int rc;
     pthread_t thread;
     int running;
+    int centerlines;  /* also write the centreline profiles, of step `step`, spacing h */
+    int step;
+    double h;
 } vtk_writer;
 
 static void *vtk_writer_main(void *arg)
@@ -95,7 +120,10 @@ static void *vtk_writer_main(void *arg)
     vtk_writer *wr = (vtk_writer *)arg;
     const double *w, *psi, *u, *v;
     wr->rc = cnv_snapshot_wait(wr->ctx, &w, &psi, &u, &v);
-    if (wr->rc == CNV_OK) write_vtk_set(w, psi, u, v, wr->n, wr->dir);
+    if (wr->rc == CNV_OK) {
+        write_vtk_set(w, psi, u, v, wr->n, wr->dir);
+        if (wr->centerlines) write_centerlines(u, v, wr->n, wr->h, wr->step, wr->dir);
+    }
     return NULL;
 }
 
@@ -109,10 +137,11 @@ static int vtk_writer_join(vtk_writer *wr)
     return wr->rc;
 }
 
-static int vtk_writer_start(vtk_writer *wr)
+static int vtk_writer_start(vtk_writer *wr, int step)
 {
     int rc = vtk_writer_join(wr);
     if (rc != CNV_OK) return rc;
+    wr->step = step;  /* the previous writer, which read it, has been joined */
     rc = cnv_snapshot_begin(wr->ctx);
     if (rc != CNV_OK) {
         printf("Error: %s\n", cnv_last_error(wr->ctx));
@@ -246,7 +275,7 @@ int main(int argc, char *argv[])
             return 1;
         }
     }
-    vtk_writer writer = {ctx, n, output_dir, CNV_OK, 0, 0};
+    vtk_writer writer = {ctx, n, output_dir, CNV_OK, 0, 0, cx.centerlines, 0, dx};
     const double t_sim = now_s();
     for (int t = 0; t < nsteps; t++) {
         const double t0 = now_s();
@@ -284,7 +313,7 @@ int main(int argc, char *argv[])
             logf_(log_file, "Est. remaining: -- s\n");
         if (cfg.output_interval > 0 && t % cfg.output_interval == 0) {
             if (cx.output_async) {
-                if (vtk_writer_start(&writer) != CNV_OK) {
+                if (vtk_writer_start(&writer, t) != CNV_OK) {
                     vtk_writer_join(&writer);
                     cnv_destroy(ctx);
                     return 1;
@@ -297,6 +326,7 @@ int main(int argc, char *argv[])
                     return 1;
                 }
                 write_vtk_set(w, psi, u, v, n, output_dir);
+                if (cx.centerlines) write_centerlines(u, v, n, dx, t, output_dir);
             }
         }
     }

@@ -85,6 +85,7 @@ static int ext_key(ConfigExt *x, const char *k, const char *v)
     if (strcmp(k, "warm_start") == 0) { x->warm_start = atoi(v); return 1; }
     if (strcmp(k, "output_async") == 0) { x->output_async = atoi(v); return 1; }
     if (strcmp(k, "transport_fp32") == 0) { x->transport_fp32 = atoi(v); return 1; }
+    if (strcmp(k, "centerlines") == 0) { x->centerlines = atoi(v); return 1; }
     if (strcmp(k, "inner_beta") == 0) {
         /* -1 marks "reference": resolved with the grid size once the whole file is read */
         x->inner_beta = strcmp(v, "reference") == 0 ? -1.0 : atof(v);
@@ -177,7 +178,7 @@ void print_usage(const char *program_name)
     printf("\nEvery step runs on the GPU.  Keys added to the reference's set:\n");
     printf("  sor_omega, nsteps, precision (fp64 | fp32dc), inner_rtol, inner_beta,\n");
     printf("  sweeps_per_launch, diagnostics, device, use_graph, warm_start, output_async,\n");
-    printf("  transport_fp32\n");
+    printf("  transport_fp32, centerlines\n");
     printf("\nFor the list of parameters, run with --help-config\n");
 }
 

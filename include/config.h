@@ -48,6 +48,10 @@ typedef struct {
                                next steps run (default); 0 = between steps, as the reference     */
     int transport_fp32;     /* transport_fp32: 1 = fp32dc also runs the transport stencils in
                                FP32 (default 0: FP64)                                            */
+    int centerlines;        /* centerlines: 1 = at each output step also write the centreline
+                               profiles u(x = 0.5, y) and v(x, y = 0.5) to
+                               <output>/centerlines-<step>.csv (default 0: the reference's files
+                               only)                                                             */
     int has_sor_omega;
 } ConfigExt;
 

@@ -46,3 +46,31 @@ def test_inner_beta_reference_key(tmp_path):
     print("inner sweeps, inner_beta = sor_omega:", plain, " reference:", relaxed, flush=True)
     assert len(plain) == len(relaxed) == 2
     assert sum(relaxed) < 0.5 * sum(plain)
+
+
+def test_centerline_profiles_match_oracle(tmp_path):
+    # centerlines = 1: at each output step the driver writes u(x = 0.5, y) = u[:, n/2] and
+    # v(x, y = 0.5) = v[n/2, :] (node j at x = j/n, src/main.c:138-139) with full precision; after
+    # output steps 0 and 5 (the states after 1 and 6 steps) they equal the oracle's red-black
+    # run bit for bit, and the reference's own files are unchanged beside them
+    import numpy as np
+    from oracle import orc
+    n, steps = 64, 8
+    cfg = ("Re = 1000.0\nnx = 64\nny = 64\ndt = 0.005\ntf = 20.0\nmax_co = 1.0\norder = 6\npoisson_max_it = 100000\n"
+           "poisson_tol = 1E-8\npoisson_type = 2\noutput_interval = 5\nnsteps = %d\ncenterlines = 1\n" % steps)
+    run(tmp_path, cfg, "cl")
+    out = tmp_path / "output" / "cl"
+    names = sorted(os.listdir(out))
+    assert [x for x in names if x.startswith("centerlines")] == ["centerlines-0.csv", "centerlines-5.csv"]
+    assert len([x for x in names if x.endswith(".vtk")]) == 8
+    ref = orc.State(orc.make_params(n, Re=1000.0, dt=0.005, order=6, tol=1e-8, itmax=100000, ordering=orc.REDBLACK))
+    per = ref.run(1)
+    for step in (0, 5):
+        if step == 5:
+            ref.run(5)
+        rows = np.loadtxt(out / ("centerlines-%d.csv" % step), delimiter=",", comments="#", skiprows=2)
+        assert rows.shape == (n, 4)
+        assert np.array_equal(rows[:, 0], np.arange(n)) and np.array_equal(rows[:, 1], np.arange(n) / n)
+        assert np.array_equal(rows[:, 2], ref.u[:, n // 2]), step
+        assert np.array_equal(rows[:, 3], ref.v[n // 2, :]), step
+    assert per[0].sweeps > 0
