@@ -14,7 +14,10 @@
 // ("fold") once its update norm fell below inner_rtol times the L1 norm of e, where FP32 stops
 // resolving the update; the correction is then folded into psi64 and the residual refreshed in
 // FP64, which restores full resolution.  The residual, the psi accumulation, the omega update and
-// the velocities (small differences of a large psi) are FP64; the transport stencils run in FP32.
+// the velocities (small differences of a large psi) are FP64.  The transport stencils are FP64 by
+// default (transport_fp32 = 0): their FP32 rounding, added to omega step after step, makes the
+// refined trajectory drift from the FP64 one; transport_fp32 = 1 runs them in FP32
+// (transport_fp32_at) for short runs.
 #include <float.h>
 
 #include "cnv_device.h"

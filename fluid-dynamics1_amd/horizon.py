@@ -8,8 +8,9 @@
 fp64 / refined: the 1024^2 cavity at Re 1000 (dt = 50*2^-20, order 6, SOR omega 1.95 to an L1 update
 below 1e-10, lid velocity 1) in plain FP64 (captured-graph steps) or in the FP32 + FP64
 defect-correction form (inner solves fold at 1e-5 of the correction's norm).  A segment runs at
-most --segment steps from --state-in (or from rest) and writes the state reached, with the
-per-step statistics of every segment so far, to --state-out.  Each step's Poisson solve starts
+most --segment steps from --state-in (or from rest) and writes the state reached (w, u, v; psi
+too once the run is complete), with the per-step statistics of every segment so far, to
+--state-out.  Each step's Poisson solve starts
 from psi = 0 (src/poisson.c:229), so a step depends only on (w, u, v): a run in segments equals
 one run, bit for bit.  Progress lines go to stderr every --print-every steps.
 
@@ -74,7 +75,10 @@ def run_cavity(args):
                       % (args.form, step, s.sweeps, ms, done.mean(), len(done), time.perf_counter() - wall0),
                       file=sys.stderr, flush=True)
         state = cav.get_state()
-    out = {f: state[f] for f in FIELDS}
+    # psi is kept only with the final state (each step recomputes it from zero, so a resumed
+    # segment does not need it)
+    final = t0_step + todo >= args.steps
+    out = {f: state[f] for f in FIELDS if f != "psi" or final}
     out.update({k: np.array(v) for k, v in hist.items()})
     out["step"] = np.array(t0_step + todo)
     out["form"] = np.array(args.form)
