@@ -157,14 +157,14 @@ int cnv_poisson_device(cnv_ctx *ctx, const double *d_f, double *d_psi, int *k, d
  * cnv_poisson_device without a context and without allocating anything.  psi = SOR(f) from zero
  * (src/poisson.c:224-285) on an n x n grid with spacing (dx, dy), relaxation beta, tolerance tol,
  * sweep budget itmax, poisson_type 1 (Gauss-Seidel) or 2 (SOR); f and psi are device pointers to
- * n*n doubles (psi is written; f is read only).  Runs on the null stream of the current device and
- * returns when psi is complete.  The solve's scratch (the second and third iterate planes, the
- * per-tile partial sums, the state slots) is a static device array of the library, sized for
- * n <= CNV_NOALLOC_MAX_N (about 17 MB, held on each device the library runs on until the process
- * ends); larger n is refused with CNV_ERR_ARG.  Calls that overlap on one device share that array
- * and race.
- * Results, k and e equal cnv_poisson_device's on a context created with the same n, dx, dy,
- * beta, tol, itmax, poisson_type and the default launch shape. */
+ * n*n doubles (psi is written; f is read only).  Runs on the null stream of the current device
+ * and returns when psi is complete; f must be complete when it is called (work queued on a
+ * non-blocking stream is not waited for).  The solve's scratch (the second and third iterate
+ * planes, the per-tile partial sums, the state slots) is a static device array of the library,
+ * sized for n <= CNV_NOALLOC_MAX_N (about 17 MB, held on each device the library runs on until
+ * the process ends); larger n is refused with CNV_ERR_ARG.  Calls that overlap on one device
+ * share that array and race.  Results, k and e equal cnv_poisson_device's on a context created
+ * with the same n, dx, dy, beta, tol, itmax, poisson_type and the default launch shape. */
 #define CNV_NOALLOC_MAX_N 1024
 int cnv_poisson_device_noalloc(int n, double dx, double dy, double beta, double tol, int itmax,
                                int poisson_type, const double *d_f, double *d_psi, int *k, double *e);
