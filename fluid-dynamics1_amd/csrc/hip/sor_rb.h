@@ -191,7 +191,7 @@ __device__ inline T* sor_plane(const SorArgs<T>& a, int idx) {
 // solve converged inside the launch before last: redo rb_n sweeps from plane rb_src into plane
 // `parity`, which then holds the final iterate; anything else -- the solve has ended: return.
 template <typename T, int S, int NW, int RB, bool GS, bool XR, bool NORM>
-__device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int tile) {
+__device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L, const int tile) {
 #pragma clang fp contract(off)
     constexpr int NB = SorPlanes<T>::value;
     constexpr int H = 2 * S;
@@ -218,7 +218,6 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int ti
     __shared__ double lred[SE][NW * 64];
     __shared__ double lnrm[kNorm ? NW * 64 : 1];
 
-    const int L = a.launch;
     const SorCtl* prev = a.ctl + ((L + 1) & 1);
     const int tid = threadIdx.x;
     const int ntiles = a.tiles_i * a.tiles_j;
@@ -423,7 +422,7 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int ti
 // tiles t, t + NT, ... in that order, each wave reduces its threads' sums with DPP moves, and lane
 // i adds the per-wave totals of quantity i in wave order.
 template <typename T, int S, int NW, bool NORM>
-__device__ __forceinline__ void sor_rb_decider(const SorArgs<T>& a) {
+__device__ __forceinline__ void sor_rb_decider(const SorArgs<T>& a, const int L) {
     constexpr int NB = SorPlanes<T>::value;
     constexpr bool kNorm = sizeof(T) == 4 || NORM;
     constexpr bool kLastOnly = sizeof(T) == 4;
@@ -431,7 +430,6 @@ __device__ __forceinline__ void sor_rb_decider(const SorArgs<T>& a) {
     constexpr int nq = SE + (kNorm ? 1 : 0);  // reduced quantities: SE updates (and the norm)
     __shared__ double lw[NW][nq];             // per-wave totals
 
-    const int L = a.launch;
     const SorCtl* prev = a.ctl + ((L + 1) & 1);
     SorCtl* mine = a.ctl + (L & 1);
     const int tid = threadIdx.x;
@@ -580,21 +578,22 @@ __device__ __forceinline__ void sor_rb_decider(const SorArgs<T>& a) {
 template <typename T, int S, int NW, int RB, bool GS, bool XR, bool NORM = false>
 __global__ void __launch_bounds__(NW * 64) sor_rb_kernel(SorArgs<T> a) {
     const int ntiles = a.tiles_i * a.tiles_j;
-    if ((int)blockIdx.x == ntiles) sor_rb_decider<T, S, NW, NORM>(a);
-    else sor_rb_sweeper<T, S, NW, RB, GS, XR, NORM>(a, sor_tile_of_block(blockIdx.x, ntiles));
+    if ((int)blockIdx.x == ntiles) sor_rb_decider<T, S, NW, NORM>(a, a.launch);
+    else sor_rb_sweeper<T, S, NW, RB, GS, XR, NORM>(a, a.launch, sor_tile_of_block(blockIdx.x, ntiles));
 }
 
 // Batched form: workgroup b runs tile map[b].tile of solve map[b].inst (inst < 0: padding; tile
-// == the solve's tile count: its decider).  All solves of a batch share the launch index L.
+// == the solve's tile count: its decider).  All solves of a batch share the launch index L.  The
+// solve's arguments are read in place from global memory: a private copy would be indexed by the
+// plane number and live in scratch memory.
 template <typename T, int S, int NW, int RB, bool GS, bool XR>
 __global__ void __launch_bounds__(NW * 64) sor_rb_batch_kernel(const SorArgs<T>* __restrict__ args,
                                                                const SorBlock* __restrict__ map, int L) {
     const SorBlock m = map[blockIdx.x];
     if (m.inst < 0) return;
-    SorArgs<T> a = args[m.inst];
-    a.launch = L;
-    if (m.tile == a.tiles_i * a.tiles_j) sor_rb_decider<T, S, NW, false>(a);
-    else sor_rb_sweeper<T, S, NW, RB, GS, XR, false>(a, m.tile);
+    const SorArgs<T>& a = args[m.inst];
+    if (m.tile == a.tiles_i * a.tiles_j) sor_rb_decider<T, S, NW, false>(a, L);
+    else sor_rb_sweeper<T, S, NW, RB, GS, XR, false>(a, L, m.tile);
 }
 
 template <typename T, int S, int NW, int RB>
