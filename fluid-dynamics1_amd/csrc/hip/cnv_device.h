@@ -106,7 +106,6 @@ struct SorArgs {
     T c1, c2, c3, B, beta, omb;  // dy*dy, dx*dx, dx*dx*dy*dy, 2*(dx*dx+dy*dy), beta, 1-beta
     T rB;                // RN(1/B)
     int exact_rcp;       // 1: 1/B is exact (B a power of two), so x*rB == RN(x/B)
-    int vec2;            // 1: ld is even and the buffers are 2*sizeof(T)-aligned (paired access)
     int norm;            // FP64: also sum the iterate's L1 norm (needed when rtol > 0)
 };
 

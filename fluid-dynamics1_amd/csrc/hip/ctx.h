@@ -85,15 +85,6 @@ int cnv_fail(cnv_ctx* c, int code, const char* fmt, ...);
                                       hipGetErrorString(hipGetLastError()));                    \
     } while (0)
 
-// SOR launch arguments of context c for the planes src (source), b0 / b1 (ping-pong).
-// Paired (2-element vector) access needs an even row length and pitch and 2*sizeof(T)-aligned
-// buffers.
-template <typename T>
-inline int sor_vec2(const cnv::SorArgs<T>& a) {
-    const uintptr_t m = 2 * sizeof(T) - 1;
-    return (a.nc % 2 == 0) && (a.ld % 2 == 0) && !((uintptr_t)a.src_f & m) && !((uintptr_t)a.psi0 & m) && !((uintptr_t)a.psi1 & m);
-}
-
 // SOR launch arguments for an n x n grid (pitch ld) with spacing (dx, dy), relaxation beta, the
 // given tiling, scratch (partial sums, state slots), source plane src and iterate planes b0, b1,
 // b2 (b2: FP64 only, nullptr for FP32).
@@ -125,7 +116,6 @@ inline cnv::SorArgs<T> sor_args_for(int n, int ld, double dx, double dy, double 
     int e2 = 0;
     const double Bd = 2 * (dx * dx + dy * dy);
     a.exact_rcp = std::frexp(Bd, &e2) == 0.5 && std::isnormal(1.0 / Bd) && (1.0 / Bd) * Bd == 1.0;
-    a.vec2 = sor_vec2(a);
     return a;
 }
 

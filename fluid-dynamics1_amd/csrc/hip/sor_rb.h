@@ -15,9 +15,11 @@
 //     are exchanged through LDS, once per half-sweep;
 //   * every point of a row is computed, so the sweep is straight-line code with no selects.  A
 //     point outside the valid rectangle takes a wrong (finite) value, which only points outside
-//     the valid rectangle ever read.  Domain-boundary points (and points outside the domain) must
-//     keep psi = 0: their relaxation factor is 0 instead of beta, which leaves them at +-0, and
-//     the final store writes +0 there;
+//     the valid rectangle ever read.  Domain-boundary points must keep psi = 0: their relaxation
+//     factor is 0 instead of beta, which leaves them at +-0, and the final store writes +0 there.
+//     Points outside the domain have factor 0 too and start from a copy of the nearest data
+//     (the tile loads are clamped into the plane): finite, and read only by the boundary and by
+//     other points outside;
 //   * the L1 norms are accumulated per lane with a 0/1 weight per row (owned rows only) and a
 //     per-column ownership mask, written to LDS once per sweep and reduced once per launch
 //     (FP32 planes: in the launch's last sweep only).
@@ -30,17 +32,19 @@
 // power of two (dx = dy = 2^-k), 1/B is exact and x*(1/B) is already RN(x/B): one multiply (XR).
 // The L1 norm of each sweep's update (src/poisson.c:34-60; sqrt(pow(d,2)) == |d|) is summed
 // over owned points per workgroup in a fixed order into partial[L&1][s][tile] (FP32 planes: the
-// last sweep's only, into s = 0).  The next launch
-// decides on them after its own (speculative) sweeps, so that neither the partials' latency nor
-// the reduction delays the sweeps: every workgroup reduces the partials in the same fixed order,
-// so all reach the same decision, and tile 0 records it in the state slot L&1 (SorCtl) that the
-// host polls:
-//   * a launch whose previous state is terminal copies it forward and returns;
-//   * if the previous launch ended the solve, this launch's speculative sweeps are discarded;
-//   * if sweep s < S-1 of the previous launch reached e < tol, that launch's output is
-//     discarded too: this launch redoes s+1 sweeps from its (untouched) input buffer, so the
-//     final iterate is exactly the one of the first sweep with e < tol (src/poisson.c:273);
-//     FP32 planes, a preconditioner, test the last sweep of each launch only;
+// last sweep's only, into s = 0).  The next launch decides on them in one extra workgroup beside
+// its sweeping ones (the decider), so that neither the partials' latency nor the reduction delays
+// any sweep: the decider of launch L+1 reduces launch L's partials in a fixed order and writes
+// state slot (L+1)&1 (SorCtl), which launch L+2's workgroups and the host read; the sweepers of
+// launch L+1 read slot L&1 only, complete before they started, and sweep speculatively:
+//   * a launch whose slot is terminal returns after its tile load (its decider copies the slot
+//     forward);
+//   * if launch L's sweeps ended the solve, launch L+1's speculative sweeps are discarded;
+//   * if sweep s < S-1 of launch L reached e < tol, launch L's output is discarded too: launch
+//     L+2 redoes s+1 sweeps from launch L's input plane, which FP64 keeps intact by rotating
+//     through three planes, so the final iterate is exactly the one of the first sweep with
+//     e < tol (src/poisson.c:273); FP32 planes, a preconditioner, test the last sweep of each
+//     launch only and rotate through two;
 //   * the sweep budget itmax is honoured exactly (src/poisson.c:234);
 //   * a non-finite L1 update (or iterate norm) ends the solve with SOR_DIVERGED;
 //   * relative rule: with rtol > 0 the solve also stops ("fold") once the update norm of a sweep
@@ -76,51 +80,36 @@ __device__ inline double sor_div(double x, double B, double rB) {
 // FP32 planes (defect correction): no bit-parity requirement, one multiply by RN(1/B)
 __device__ inline float sor_div(float x, float, float rB) { return x * rB; }
 
-// The lane's column pair (gj, gj+1) of one row, gj even.  vec (wave-uniform, checked by the
-// host: nc and ld even, buffers 2*sizeof(T)-aligned): one 2-element access, the pair's when it
-// lies inside the row (pair), with the columns outside in0 / in1 read as 0 or left unwritten.
-// Otherwise per column.  Loads outside the data fetch element 0 instead and discard it.
+// Two consecutive elements of a row, aligned to one element: rows of any pitch are accessed in
+// pairs, one multi-dword access each (the hardware needs dword alignment only).
+template <typename T> struct PairOf;
+template <> struct PairOf<double> { typedef double type __attribute__((ext_vector_type(2), aligned(8))); };
+template <> struct PairOf<float> { typedef float type __attribute__((ext_vector_type(2), aligned(4))); };
+
+// The pair at byte offset boff of a row: the row pointer is wave-uniform and the lane's offset a
+// 32-bit value, so the address is a scalar base plus a vector offset with no 64-bit lane arithmetic.
 template <typename T>
-__device__ inline void load_pair(const T* __restrict__ p, long long off, bool pair, bool in0, bool in1,
-                                 bool vec, T& x0, T& x1) {
-    typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type V2;
-    // branch-free: lanes without data read element 0 (always valid) and discard it, so no
-    // divergent branch makes the compiler wait for each load before the next one is issued
-    if (vec) {
-        const V2 d = *reinterpret_cast<const V2*>(p + (pair ? off : 0));
-        x0 = in0 ? d.x : T(0);
-        x1 = in1 ? d.y : T(0);
-    } else {
-        const T d0 = p[in0 ? off : 0];
-        const T d1 = p[in1 ? off + 1 : 0];
-        x0 = in0 ? d0 : T(0);
-        x1 = in1 ? d1 : T(0);
-    }
+__device__ inline typename PairOf<T>::type load_pair(const T* __restrict__ row, unsigned boff) {
+    return *reinterpret_cast<const typename PairOf<T>::type*>(reinterpret_cast<const char*>(row) + boff);
 }
 // FP64 tiles are stored non-temporally: the tile and source just loaded fill much of L2, and the
 // owned tile's 9 MB (at 1024^2) would otherwise be written back from it in a burst as the
 // kernel ends (measured 0.4 us per launch faster at 1024^2; no gain for the FP32 planes).
+// both: the pair (x0, x1), else x0 alone.
 template <typename T>
-__device__ inline void store_pair(T* __restrict__ p, long long off, bool in0, bool in1, bool vec, T x0,
-                                  T x1) {
-    typedef T V2 __attribute__((ext_vector_type(2)));
+__device__ inline void store_pair(T* __restrict__ row, unsigned boff, bool both, T x0, T x1) {
+    typedef typename PairOf<T>::type P;
     constexpr bool kNT = sizeof(T) == 8;
-    if (vec) {  // nc even: in0 == in1 for an even gj
-        if (in0) {
-            V2 d;
-            d.x = x0;
-            d.y = x1;
-            if (kNT) __builtin_nontemporal_store(d, reinterpret_cast<V2*>(p + off));
-            else *reinterpret_cast<V2*>(p + off) = d;
-        }
+    T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(row) + boff);
+    if (both) {
+        P d;
+        d.x = x0;
+        d.y = x1;
+        if (kNT) __builtin_nontemporal_store(d, reinterpret_cast<P*>(p));
+        else *reinterpret_cast<P*>(p) = d;
     } else {
-        if (kNT) {
-            if (in0) __builtin_nontemporal_store(x0, p + off);
-            if (in1) __builtin_nontemporal_store(x1, p + off + 1);
-        } else {
-            if (in0) p[off] = x0;
-            if (in1) p[off + 1] = x1;
-        }
+        if (kNT) __builtin_nontemporal_store(x0, p);
+        else *p = x0;
     }
 }
 
@@ -225,32 +214,49 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
     const int tj = tile - ti * a.tiles_j;
     const int gi0 = ti * TI - H;  // even, so the colour of local (li, lj) is (li + lj) & 1
     const int gj0 = tj * TJ - H;
-    const int nr = a.nr, nc = a.nc, ld = a.ld;
-    const bool vec = a.vec2 != 0;
+    const int nr = a.nr, nc = a.nc;
+    const long long ld = a.ld;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (SGPR)
     const int row0 = wv * RB;        // even: the colour of (row0 + r, 2m + q) is (r + q) & 1
 
-    // Speculative tile load for the common case (sweep on from the previous launch's output),
-    // issued before the state slot is read so that both are in flight together.  Column facts:
-    // q = 0 -> column 2m, q = 1 -> column 2m+1.
-    const T* __restrict__ srcN = sor_plane(a, L % NB);
-    bool colin[2], ownc[2], colv[2];
-    int gjq[2];
+    // Column facts of the lane's pair (gj, gj+1), gj even: q = 0 -> column gj, q = 1 -> gj+1.
+    const int gj = gj0 + 2 * lane;
+    bool colin[2], ownc[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const int lj = 2 * lane + q;
-        gjq[q] = gj0 + lj;
-        colv[q] = (unsigned)gjq[q] < (unsigned)nc;
-        colin[q] = gjq[q] >= 1 && gjq[q] <= nc - 2;
-        ownc[q] = lj >= H && lj < H + TJ && gjq[q] < nc;
+        colin[q] = gj + q >= 1 && gj + q <= nc - 2;
+        ownc[q] = 2 * lane + q >= H && 2 * lane + q < H + TJ && gj + q < nc;  // ownc[1] implies ownc[0]
     }
+    // Every load reads a pair inside the data: rows clamped into [0, nr), the pair's first column
+    // into [0, nc-2].  So no lane needs a select on what it loads: a point outside the domain may
+    // start from any finite value (it is read only by points outside the domain and by the domain
+    // boundary, whose relaxation factor is 0), and each domain-boundary point is read from its own
+    // position (+-0 there).  With nc odd, the lane of column nc-1 reads the pair (nc-2, nc-1) and
+    // takes its second element (fixed after the loads); its column nc is outside.
+    const unsigned bld = (unsigned)(gj < 0 ? 0 : (gj > nc - 2 ? nc - 2 : gj)) * sizeof(T);
+    auto row_of = [&](const T* plane, int gi) -> const T* {  // wave-uniform row pointer
+        return plane + (gi < 0 ? 0 : (gi >= nr ? nr - 1 : gi)) * ld;
+    };
+
     // relaxation factor of an updated point: FP64 beta (1 for Gauss-Seidel), applied before the
-    // division; FP32 beta/B (1/B), which folds the division in
+    // division; FP32 beta/B (1/B), which folds the division in.  Per column, 0 off the interior
+    // columns; per point, also 0 on the boundary rows and outside the domain.
     T bcoef;
     if constexpr (sizeof(T) == 8) bcoef = GS ? T(1) : a.beta;
     else bcoef = GS ? a.rB : a.beta * a.rB;
+    const T c3s = a.negate ? -a.c3 : a.c3;  // f = -src folded in: c3*(-x) == (-c3)*x exactly
+    T bcol[2], cncol[2], cecol[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        bcol[q] = colin[q] ? bcoef : T(0);
+        cncol[q] = colin[q] ? bcoef * a.c1 : T(0);
+        cecol[q] = colin[q] ? bcoef * a.c2 : T(0);
+    }
 
+    // Speculative tile load for the common case (sweep on from the previous launch's output),
+    // issued before the state slot is read so that both are in flight together.
+    const T* __restrict__ srcN = sor_plane(a, L % NB);
     T v[2][RB];   // v[q][r]: value at (row0 + r, 2*lane + q)
     T fc[2][RB];  // FP64: c3*f (the hoisted c3*f product of the update); FP32: -be*c3*f
     T be[2][RB];  // FP64: bcoef at interior points, 0 on the domain boundary and outside it
@@ -261,26 +267,24 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
     for (int r = 0; r < RB; ++r) {
         const int li = row0 + r;
         const int gi = gi0 + li;
-        const bool rin = (unsigned)gi < (unsigned)nr;
         const bool rint = gi >= 1 && gi <= nr - 2;
         w[r] = (li >= H && li < H + TI && gi < nr) ? Acc(1) : Acc(0);
-        const long long off = (long long)(rin ? gi : 0) * ld + gjq[0];
-        T f0, f1;
-        load_pair(srcN, off, rin && colv[0], rin && colv[0], rin && colv[1], vec, v[0][r], v[1][r]);
-        load_pair(a.src_f, off, rint && colv[0], rint && colin[0], rint && colin[1], vec, f0, f1);
+        const typename PairOf<T>::type d = load_pair(row_of(srcN, gi), bld);
+        const typename PairOf<T>::type f = load_pair(row_of(a.src_f, gi), bld);
+        v[0][r] = d.x;
+        v[1][r] = d.y;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
+            // a non-interior point's source is not read by the reference: 0, whatever it holds
             const bool inter = rint && colin[q];
-            const T fq = q ? f1 : f0;
-            const T fv = a.negate ? -fq : fq;
-            const T bq = inter ? bcoef : T(0);
+            const T fq = q ? f.y : f.x;
             if constexpr (sizeof(T) == 8) {
-                fc[q][r] = a.c3 * fv;
-                be[q][r] = bq;
+                fc[q][r] = inter ? c3s * fq : T(0);
+                be[q][r] = rint ? bcol[q] : T(0);
             } else {
-                fc[q][r] = bq * -(a.c3 * fv);
-                cn[q][r] = bq * a.c1;
-                ce[q][r] = bq * a.c2;
+                fc[q][r] = inter ? bcoef * -(c3s * fq) : T(0);
+                cn[q][r] = rint ? cncol[q] : T(0);
+                ce[q][r] = rint ? cecol[q] : T(0);
             }
         }
     }
@@ -305,10 +309,15 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
         const T* __restrict__ srcR = sor_plane(a, prev->rb_src);
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
-            const int gi = gi0 + row0 + r;
-            const bool rin = (unsigned)gi < (unsigned)nr;
-            load_pair(srcR, (long long)(rin ? gi : 0) * ld + gjq[0], rin && colv[0], rin && colv[0],
-                      rin && colv[1], vec, v[0][r], v[1][r]);
+            const typename PairOf<T>::type d = load_pair(row_of(srcR, gi0 + row0 + r), bld);
+            v[0][r] = d.x;
+            v[1][r] = d.y;
+        }
+    }
+    if (nc & 1) {  // wave-uniform: the lane of column nc-1 read the pair (nc-2, nc-1)
+        if (gj == nc - 1) {
+#pragma unroll
+            for (int r = 0; r < RB; ++r) v[0][r] = v[1][r];
         }
     }
 
@@ -393,8 +402,10 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
         const int gi = gi0 + li;
         if (li >= H && li < H + TI && gi < nr) {  // owned row (wave-uniform)
             const bool rint = gi >= 1 && gi <= nr - 2;
-            store_pair(dst, (long long)gi * ld + gjq[0], ownc[0], ownc[1], vec,
-                       (rint && colin[0]) ? v[0][r] : T(0), (rint && colin[1]) ? v[1][r] : T(0));
+            // owned columns lie inside the row: 0 <= gj < nc
+            if (ownc[0])
+                store_pair(dst + gi * ld, (unsigned)gj * sizeof(T), ownc[1], (rint && colin[0]) ? v[0][r] : T(0),
+                           (rint && colin[1]) ? v[1][r] : T(0));
         }
     }
     if (rollback) return;  // the final iterate: no decision follows
