@@ -330,8 +330,10 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
         __syncthreads();
     };
     // sweep s (red half-sweep, then black) of a pass; with_norm: also sum the iterate's norm
-    // (FP32: and the update, which only the last sweep sums)
-    auto one_sweep = [&](const int s, const bool with_norm) {
+    // (FP32: and the update, which only the last sweep sums); l1 (std::true_type or false_type):
+    // sum the L1 norms at all, which a wave holding no owned row skips (its sums are 0)
+    auto one_sweep = [&](const int s, const bool with_norm, auto l1) {
+        constexpr bool kL1 = decltype(l1)::value;
         Acc acc[2] = {Acc(0), Acc(0)}, nrm[2] = {Acc(0), Acc(0)};
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
@@ -358,14 +360,16 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
                     const T qt = XR ? y * rB : sor_div(y, B, rB);
                     cand = GS ? qt : qt + omb * old;
                     v[q][r] = cand;
-                    acc[q] = __fma_rn(w[r], fabs(cand - old), acc[q]);
-                    if (kNorm && with_norm) nrm[q] = __fma_rn(w[r], fabs(cand), nrm[q]);
+                    if constexpr (kL1) {
+                        acc[q] = __fma_rn(w[r], fabs(cand - old), acc[q]);
+                        if (kNorm && with_norm) nrm[q] = __fma_rn(w[r], fabs(cand), nrm[q]);
+                    }
                 } else {
                     // relaxation and 1/B folded into cn, ce and fc (0 where the point is fixed)
                     const T x = __fmaf_rn(cn[q][r], up + dn, __fmaf_rn(ce[q][r], own + far, fc[q][r]));
                     cand = GS ? x : __fmaf_rn(omb, old, x);
                     v[q][r] = cand;
-                    if (with_norm) {
+                    if (kL1 && with_norm) {
                         acc[q] = __fmaf_rn(w[r], fabsf(cand - old), acc[q]);
                         nrm[q] = __fmaf_rn(w[r], fabsf(cand), nrm[q]);
                     }
@@ -386,13 +390,23 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
     // FP64 shapes whose unrolled pass no longer fits the register budget of their wave count.
     // A rollback or a budget-limited last launch takes the loop, which sums the FP64 norm in
     // every sweep (the last one's stays).
+    // The waves that hold only halo rows (a quarter of them at 1024^2) run it without the FP64
+    // L1 sums, two of the point update's 14 instructions.
     constexpr bool kUnroll = sizeof(T) == 4 || (S * RB <= 40 && NW != 12 && !(kNorm && S * RB > 20));
+    bool owns = false;  // wave-uniform: the wave holds an owned row
+#pragma unroll
+    for (int r = 0; r < RB; ++r) owns = owns || w[r] != Acc(0);
     sweeps_begin();
     if (kUnroll && s_run == S) {
+        if (owns || sizeof(T) == 4) {
 #pragma unroll
-        for (int s = 0; s < S; ++s) one_sweep(s, s == S - 1);
+            for (int s = 0; s < S; ++s) one_sweep(s, s == S - 1, std::true_type{});
+        } else {
+#pragma unroll
+            for (int s = 0; s < S; ++s) one_sweep(s, s == S - 1, std::false_type{});
+        }
     } else {
-        for (int s = 0; s < s_run; ++s) one_sweep(s, kLastOnly ? s == s_run - 1 : kNorm);
+        for (int s = 0; s < s_run; ++s) one_sweep(s, kLastOnly ? s == s_run - 1 : kNorm, std::true_type{});
     }
 
     // the owned tile to its plane; domain-boundary points hold +-0: store +0
@@ -418,10 +432,15 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
     const int nrows = nupd + (kNorm ? 1 : 0);
     for (int row = wv; row < nrows; row += NW) {
         const double* col = row < nupd ? lred[row] : lnrm;
-        double acc = 0.0;
+        // the NW lane sums of this column, pairwise (depth log2 NW, not a chain of NW adds)
+        double t[NW];
 #pragma unroll
-        for (int j = 0; j < NW; ++j) acc += col[lane + 64 * j];
-        acc = wave_sum(acc);
+        for (int j = 0; j < NW; ++j) t[j] = col[lane + 64 * j];
+#pragma unroll
+        for (int d = 1; d < NW; d *= 2)
+#pragma unroll
+            for (int j = 0; j + d < NW; j += 2 * d) t[j] += t[j + d];
+        const double acc = wave_sum(t[0]);
         const size_t slot = row < nupd ? ((size_t)row * ntiles + tile) * 2 : (size_t)tile * 2 + 1;
         if (lane == 0) a.partial[(L & 1) * pstride + slot] = acc;
     }
