@@ -92,24 +92,20 @@ template <typename T>
 __device__ inline typename PairOf<T>::type load_pair(const T* __restrict__ row, unsigned boff) {
     return *reinterpret_cast<const typename PairOf<T>::type*>(reinterpret_cast<const char*>(row) + boff);
 }
-// FP64 tiles are stored non-temporally: the tile and source just loaded fill much of L2, and the
-// owned tile's 9 MB (at 1024^2) would otherwise be written back from it in a burst as the
-// kernel ends (measured 0.4 us per launch faster at 1024^2; no gain for the FP32 planes).
-// both: the pair (x0, x1), else x0 alone.
+// The pair (x0, x1) at byte offset boff of a row when both, else x0 alone.  Ordinary stores in
+// both precisions: non-temporal FP64 stores measured 2.5% slower per 1024^2 step, because the
+// next launch then reads the tile from further out.
 template <typename T>
 __device__ inline void store_pair(T* __restrict__ row, unsigned boff, bool both, T x0, T x1) {
     typedef typename PairOf<T>::type P;
-    constexpr bool kNT = sizeof(T) == 8;
     T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(row) + boff);
     if (both) {
         P d;
         d.x = x0;
         d.y = x1;
-        if (kNT) __builtin_nontemporal_store(d, reinterpret_cast<P*>(p));
-        else *reinterpret_cast<P*>(p) = d;
+        *reinterpret_cast<P*>(p) = d;
     } else {
-        if (kNT) __builtin_nontemporal_store(x0, p);
-        else *p = x0;
+        *p = x0;
     }
 }
 
