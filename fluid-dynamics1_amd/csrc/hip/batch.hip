@@ -115,7 +115,8 @@ __global__ void __launch_bounds__(kBlock) batch_poisson_init_kernel(const StepIt
     if (m.lb == 0 && threadIdx.x == 0) diag_reset(I.diag);
 }
 
-template <bool DC>
+// DC: the refined form's accumulator psi[0]; F32: its FP32 stencils (velocity_fp32_at)
+template <bool DC, bool F32>
 __global__ void __launch_bounds__(kBlock) batch_velocity_kernel(const StepItem* __restrict__ it,
                                                                 const BlockRef* __restrict__ map, int fin_slot) {
     const BlockRef m = map[blockIdx.x];
@@ -124,7 +125,8 @@ __global__ void __launch_bounds__(kBlock) batch_velocity_kernel(const StepItem* 
     int i, j;
     if (!cell_of(m, I.n, i, j)) return;
     const double* psi = DC ? I.psi[0] : I.psi[I.ctl[fin_slot].parity];
-    velocity_at(psi, I.u, I.v, I.ld, I.st, i, j);
+    if (F32) velocity_fp32_at(psi, I.u, I.v, I.ld, I.st, i, j);
+    else velocity_at(psi, I.u, I.v, I.ld, I.st, i, j);
 }
 
 __global__ void __launch_bounds__(kBlock) batch_continuity_kernel(const StepItem* __restrict__ it,
@@ -241,7 +243,7 @@ struct cnv_batch {
     std::vector<int> alive;        // 1: the cavity takes part in the steps; 0: it diverged
     std::vector<double> blowup;    // per cavity: blowup_speed (0 = no check)
     bool maps_dirty = false;       // alive changed: rebuild the workgroup maps before the next step
-    bool transport_fp32 = false;  // refined form: FP32 transport stencils (default FP64)
+    bool transport_fp32 = true;  // refined form: FP32 transport and velocity stencils (the default)
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<cnv_ctx*> ctx;
@@ -546,12 +548,15 @@ static int batch_one_step(cnv_batch* b, cnv_step_stats* stats) {
         if (b->alive[i]) kmax = std::max(kmax, b->dc ? 0 : fin[i].k);
     if (!b->dc) b->k_prev = kmax;
 
-    if (b->dc)
-        hipLaunchKernelGGL(batch_velocity_kernel<true>, dim3(b->nb_cell), dim3(kBlock), 0, b->stream, b->d_items,
-                           b->d_map_cell, b->fin_slot);
+    if (b->dc && b->transport_fp32)
+        hipLaunchKernelGGL((batch_velocity_kernel<true, true>), dim3(b->nb_cell), dim3(kBlock), 0, b->stream,
+                           b->d_items, b->d_map_cell, b->fin_slot);
+    else if (b->dc)
+        hipLaunchKernelGGL((batch_velocity_kernel<true, false>), dim3(b->nb_cell), dim3(kBlock), 0, b->stream,
+                           b->d_items, b->d_map_cell, b->fin_slot);
     else
-        hipLaunchKernelGGL(batch_velocity_kernel<false>, dim3(b->nb_cell), dim3(kBlock), 0, b->stream, b->d_items,
-                           b->d_map_cell, b->fin_slot);
+        hipLaunchKernelGGL((batch_velocity_kernel<false, false>), dim3(b->nb_cell), dim3(kBlock), 0, b->stream,
+                           b->d_items, b->d_map_cell, b->fin_slot);
     BLAUNCH(b);
     hipLaunchKernelGGL(batch_continuity_kernel, dim3(b->nb_lin), dim3(kBlock), 0, b->stream, b->d_items, b->d_map_lin);
     BLAUNCH(b);

@@ -157,6 +157,9 @@ int snapshot_launch(const double* w, const Planes& psi, const double* u, const d
                     double* dst, long long count, hipStream_t st);
 
 // defect correction
+// refined-form velocities from psi with FP32 stencils
+int velocity_fp32_launch(const double* psi, double* u, double* v, int n, int ld, const Stencils& s,
+                         hipStream_t st);
 int dc_residual_launch(const double* w, const double* psi, float* r32, int n, int ld, double c1,
                        double c2, double c3, double B, hipStream_t st);
 int dc_correct_launch(double* psi, const float* e0, const float* e1, const SorCtl* fin, int n,

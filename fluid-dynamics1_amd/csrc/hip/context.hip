@@ -70,6 +70,7 @@ void cnv_default_params(cnv_params* p, int n) {
     p->u4 = 1.0;
     p->inner_rtol = 1e-5;
     p->max_outer = 50;
+    p->transport_fp32 = 1;
     p->diagnostics = 1;
 }
 
@@ -514,7 +515,7 @@ static int body_correct(cnv_ctx* c, int) {
 // velocities from the final iterate (slot 1 after whole chunks; psi[0] in the refined form),
 // continuity diagnostic
 static int body_post(cnv_ctx* c, int) {
-    LAUNCH(c, velocity_launch(planes_of(c), c->ctl + 1, c->u, c->v, c->n, c->ld, c->st, c->stream));
+    LAUNCH(c, velocities(c, c->ctl + 1));
     if (wants_diag(c)) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, c->n, c->ld, c->st, c->stream));
     return CNV_OK;
 }
@@ -676,12 +677,7 @@ static int one_step(cnv_ctx* c, cnv_step_stats* stats) {
     int outer = 0;
     int rc = dc ? poisson_dc(c, w_out, &fin, &outer) : poisson_fp64(c, w_out, 1, &fin);
     if (rc) return rc;
-    // velocities in FP64 in both forms: v = -(DX psi) takes small differences of a large psi,
-    // which FP32 stencils would resolve only to ~1e-7 of |psi|/h
-    if (dc)
-        LAUNCH(c, velocity_launch(planes_of(c), c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
-    else
-        LAUNCH(c, velocity_launch(planes_of(c), c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
+    LAUNCH(c, velocities(c, c->ctl + c->fin_slot));
     if (wants_diag(c)) LAUNCH(c, continuity_launch(c->u, c->v, c->diag, n, ld, c->st, c->stream));
     return finish_step(c, fin, outer, stats);
 }

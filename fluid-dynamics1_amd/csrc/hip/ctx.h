@@ -131,11 +131,21 @@ inline cnv::SorArgs<float> sor_args32(cnv_ctx* c) {
     return sor_args_for<float>(c->n, c->ld, c->p.dx, c->p.dy, beta, c->tiles_i, c->tiles_j, c->partial, c->ctl,
                                c->r32, c->e32[0], c->e32[1], nullptr, 0);
 }
+// The stencils of the refined form's transport and velocities run in FP32 (transport_fp32).
+inline bool fp32_stencils(const cnv_ctx* c) {
+    return c->p.precision == CNV_PREC_FP32_DC && c->p.transport_fp32;
+}
 // The planes of the context's solve for the kernels that read its final iterate: FP64 the three
 // SOR planes; the refined form's final iterate is always the FP64 accumulator psi[0].
 inline cnv::Planes planes_of(const cnv_ctx* c) {
     if (c->p.precision == CNV_PREC_FP32_DC) return cnv::Planes{{c->psi[0], c->psi[0], c->psi[0]}};
     return cnv::Planes{{c->psi[0], c->psi[1], c->psi[2]}};
+}
+// Velocities from the solve's final iterate (slot fin): FP64 stencils, or FP32 ones from the refined
+// form's FP64 accumulator psi[0].
+inline int velocities(cnv_ctx* c, const cnv::SorCtl* fin) {
+    if (fp32_stencils(c)) return cnv::velocity_fp32_launch(c->psi[0], c->u, c->v, c->n, c->ld, c->st, c->stream);
+    return cnv::velocity_launch(planes_of(c), fin, c->u, c->v, c->n, c->ld, c->st, c->stream);
 }
 
 // The default SOR launch shape of an n x n grid on a device with `cus` compute units.

@@ -78,8 +78,23 @@ __global__ void __launch_bounds__(kBlock) transport_fp32_kernel(const double* __
     if (i < n && j < n) transport_fp32_at(w, wout, u, v, n, ld, s, b, rRe, dt, i, j);
 }
 
+// refined-form velocities from the FP64 accumulator with FP32 stencils (velocity_fp32_at)
+__global__ void __launch_bounds__(kBlock) velocity_fp32_kernel(const double* __restrict__ psi, double* __restrict__ u,
+                                                              double* __restrict__ v, int n, int ld, Stencils s) {
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i < n && j < n) velocity_fp32_at(psi, u, v, ld, s, i, j);
+}
+
 static inline int launched() { return hipGetLastError() == hipSuccess ? 0 : 1; }
 static inline unsigned blocks_for(long long count) { return (unsigned)((count + kBlock - 1) / kBlock); }
+
+int velocity_fp32_launch(const double* psi, double* u, double* v, int n, int ld, const Stencils& s,
+                         hipStream_t st) {
+    dim3 grid((n + 63) / 64, (n + 3) / 4);
+    hipLaunchKernelGGL(velocity_fp32_kernel, grid, dim3(kBlock), 0, st, psi, u, v, n, ld, s);
+    return launched();
+}
 
 int dc_residual_launch(const double* w, const double* psi, float* r32, int n, int ld, double c1,
                        double c2, double c3, double B, hipStream_t st) {

@@ -208,23 +208,38 @@ __device__ inline float bcf(const double* a, int i, int j, int n, int ld, double
     if (i == n - 1) return (float)e3;
     return (float)a[(size_t)i * ld + j];
 }
+// Each window's coefficients sum to zero (a derivative of a constant is 0), so the stencil of f
+// equals the stencil of f - f(i, j): the taps enter as FP64 differences from the point's own value,
+// rounded to FP32, and the sums run in FP32.  Rounding f itself to FP32 would leave an error of
+// FP32 epsilon times |f| / h in every derivative (the differences cancel the leading digits), which
+// accumulates step after step; rounding the differences leaves FP32 epsilon times the local change.
 __device__ inline float fsum_x(const float* coef, const int* start, const double* f, int i, int j, int ld) {
 #pragma clang fp contract(fast)
     const float* c = coef + (size_t)j * kStencilWidth;
     const double* row = f + (size_t)i * ld + start[j];
+    const double fc = f[(size_t)i * ld + j];
     float s = 0.0f;
 #pragma unroll
-    for (int m = 0; m < kStencilWidth; ++m) s += c[m] * (float)row[m];
+    for (int m = 0; m < kStencilWidth; ++m) s += c[m] * (float)(row[m] - fc);
     return s;
 }
 __device__ inline float fsum_y(const float* coef, const int* start, const double* f, int i, int j, int ld) {
 #pragma clang fp contract(fast)
     const float* c = coef + (size_t)i * kStencilWidth;
     const double* col = f + (size_t)start[i] * ld + j;
+    const double fc = f[(size_t)i * ld + j];
     float s = 0.0f;
 #pragma unroll
-    for (int m = 0; m < kStencilWidth; ++m) s += c[m] * (float)col[(size_t)m * ld];
+    for (int m = 0; m < kStencilWidth; ++m) s += c[m] * (float)(col[(size_t)m * ld] - fc);
     return s;
+}
+// velocities of the refined form, u = DY psi, v = -(DX psi), with the stencils in FP32 (taps as
+// FP64 differences from the point's own psi, so the result keeps FP32 precision relative to the
+// derivative, not to |psi| / h)
+__device__ inline void velocity_fp32_at(const double* psi, double* __restrict__ u, double* __restrict__ v,
+                                        int ld, const Stencils& s, int i, int j) {
+    u[(size_t)i * ld + j] = (double)fsum_y(s.f1y, s.start, psi, i, j, ld);
+    v[(size_t)i * ld + j] = -(double)fsum_x(s.f1x, s.start, psi, i, j, ld);
 }
 // omega update: derivatives and right-hand side in FP32, the state update w + dt*rhs in FP64
 __device__ inline void transport_fp32_at(const double* __restrict__ w, double* __restrict__ wout,

@@ -56,7 +56,8 @@ static void help_config(void)
     printf("            sweeps_per_launch (0 = tuned default), diagnostics (1), device (0), use_graph (0),\n");
     printf("            warm_start (0; 1 = each Poisson solve starts from the previous psi, not 0),\n");
     printf("            output_async (1 = VTK files written by a second thread while the next steps run),\n");
-    printf("            transport_fp32 (0; 1 = fp32dc runs the transport stencils in FP32 as well)\n");
+    printf("            transport_fp32 (1 = fp32dc runs the transport and velocity stencils in FP32;\n");
+    printf("            0 = in FP64)\n");
 }
 
 static mtrx to_mtrx(const double *a, int n)

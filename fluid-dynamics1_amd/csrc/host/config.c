@@ -133,6 +133,7 @@ ConfigExt load_config_ext(const char *filename)
     x.inner_rtol = 1e-5;
     x.diagnostics = 1;
     x.output_async = 1;
+    x.transport_fp32 = 1;
     if (filename) parse_file(filename, &x.base, &x);
     return x;
 }

@@ -75,11 +75,12 @@ typedef struct {
                               instead of 0 (src/poisson.c:229 starts from 0): fewer sweeps,
                               results that differ within the tolerance; off for parity.
                               One-cavity contexts only (a batch refuses it)                  */
-    int transport_fp32;    /* FP32_DC: 1 = the transport stencils in FP32 too.  Default 0: FP64
-                              transport (about 25 us of a 77 ms step at 1024^2); with FP32
-                              stencils the refined trajectory drifts from the FP64 one by their
-                              rounding, step after step (2e-6 relative after 600 steps at 256^2,
-                              Re 1000), against 1e-10 with FP64 transport                     */
+    int transport_fp32;    /* FP32_DC: 1 (default) = the transport and velocity stencils run in
+                              FP32, each tap entering as an FP64 difference from the point's own
+                              value (so their rounding is relative to the derivative, and the
+                              refined trajectory does not drift from the FP64 one: about 2e-8
+                              relative over 600 steps at 256^2, Re 1000); the state update
+                              stays FP64.  0 = FP64 stencils (1e-10 there).  Ignored by FP64  */
     double blowup_speed;   /* > 0: a step after which some |u| or |v| exceeds this (or is NaN)
                               ends with CNV_DIVERGED (the explicit scheme has gone unstable);
                               0 = off, as in the reference.  In a batch the cavity is dropped

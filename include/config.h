@@ -46,8 +46,8 @@ typedef struct {
     int warm_start;         /* warm_start: 1 = each Poisson solve starts from the previous psi  */
     int output_async;       /* output_async: 1 = VTK files written by a second thread while the
                                next steps run (default); 0 = between steps, as the reference     */
-    int transport_fp32;     /* transport_fp32: 1 = fp32dc also runs the transport stencils in
-                               FP32 (default 0: FP64)                                            */
+    int transport_fp32;     /* transport_fp32: 1 (default) = fp32dc runs the transport and
+                               velocity stencils in FP32; 0 = in FP64                            */
     int centerlines;        /* centerlines: 1 = at each output step also write the centreline
                                profiles u(x = 0.5, y) and v(x, y = 0.5) to
                                <output>/centerlines-<step>.csv (default 0: the reference's files

@@ -56,6 +56,21 @@ def test_diff_tables_bits():
         lb.free(D)
 
 
+def test_diff_rows_sum_to_zero():
+    # the FP32 transport stencils take each tap as a difference from the point's own value, which
+    # equals the plain stencil only because every row of Diff1 and Diff2 sums to zero (the
+    # derivative of a constant), boundary rows included; rounding leaves a few ulps of the row's
+    # absolute sum
+    for o in (2, 4, 6):
+        for n in (12, 33, 64):
+            for fn in (lb.lib().Diff1, lb.lib().Diff2):
+                D = fn(n, o, 1.0 / (n - 1))
+                A = lb.from_m(D)
+                lb.free(D)
+                tol = 16 * np.finfo(float).eps * np.abs(A).sum(axis=1)
+                assert np.all(np.abs(A.sum(axis=1)) <= tol), (o, n, np.abs(A.sum(axis=1)).max())
+
+
 def test_linearalg_matches_reference():
     g = load("host_linearalg.npz")
     A, B = lb.to_m(g["mul_A"]), lb.to_m(g["mul_B"])

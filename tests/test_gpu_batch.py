@@ -46,10 +46,11 @@ def test_batch_equals_single(precision, steps, shape):
     check_batch_equals_single(ps, steps)
 
 
-def test_batch_fp32_transport_equals_single():
-    # the refined form with transport_fp32 = 1: the batched transport kernel's FP32 instantiation
-    ps = mix_params(cnv.PREC_FP32_DC, transport_fp32=1, sweeps_per_launch=5, tile_waves=16, wave_rows=4)
-    print("batch of", [p.n for p in ps], "refined, FP32 transport", flush=True)
+def test_batch_fp64_stencils_equals_single():
+    # the refined form with transport_fp32 = 0: the batched transport and velocity kernels' FP64
+    # instantiations (the other batch tests run the default FP32 stencils)
+    ps = mix_params(cnv.PREC_FP32_DC, transport_fp32=0, sweeps_per_launch=5, tile_waves=16, wave_rows=4)
+    print("batch of", [p.n for p in ps], "refined, FP64 stencils", flush=True)
     check_batch_equals_single(ps, 2)
 
 

@@ -37,11 +37,12 @@ def test_graph_multi_step_call():
         assert all(np.array_equal(a[f], b[f]) for f in ("w", "psi", "u", "v"))
 
 
-@pytest.mark.parametrize("warm,t32", [(0, 0), (1, 0), (0, 1)], ids=["cold", "warm", "transport32"])
+@pytest.mark.parametrize("warm,t32", [(0, 1), (1, 1), (0, 0)], ids=["cold", "warm", "stencils64"])
 def test_graph_refined_step_equals_eager(warm, t32):
     # the refined form: per outer iteration a residual graph, replayed FP32 inner chunks (big and
-    # tail) and a correction graph, the sweep budget on the device; transport in FP64 (default) or
-    # FP32 (transport_fp32 = 1), captured in the pre-solve graph
+    # tail) and a correction graph, the sweep budget on the device; transport and velocity
+    # stencils in FP32 (default) or FP64 (transport_fp32 = 0), captured in the pre- and
+    # post-solve graphs
     kw = dict(Re=1000.0, dt=1e-3, tol=1e-10, itmax=2_000_000, beta=1.9, precision=cnv.PREC_FP32_DC,
               warm_start=warm, transport_fp32=t32, ui=0.1, vi=-0.05, u=(0.2, -0.1, 0.05, 1.2),
               v=(-0.3, 0.1, 0.25, -0.15))
