@@ -1,6 +1,7 @@
 """Every compiled SOR launch shape (sweeps per launch S, waves per workgroup, rows per wave) on the
-single-cavity path, bit-exact against the oracle: steps with wall velocities on a grid whose rows
-allow 16-byte pair accesses (n even) and one that does not (n odd), and red-black Gauss-Seidel;
+single-cavity path, bit-exact against the oracle: steps with wall velocities on a grid with n
+even and one with n odd (where the lane of column n-1 reads the pair before it and takes its
+second element), and red-black Gauss-Seidel;
 the refined (FP32 + FP64) form of each shape against the FP64 result; a shape that is not
 compiled is refused."""
 import numpy as np
