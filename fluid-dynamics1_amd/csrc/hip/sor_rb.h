@@ -405,6 +405,9 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
         for (int s = 0; s < s_run; ++s) one_sweep(s, kLastOnly ? s == s_run - 1 : kNorm, std::true_type{});
     }
 
+    // every wave's lane sums are in LDS; the stores below do not wait for this barrier's peers,
+    // so the reducing waves start as soon as their own stores are issued
+    if (!rollback) __syncthreads();
     // the owned tile to its plane; domain-boundary points hold +-0: store +0
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
@@ -422,7 +425,6 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
 
     // this tile's per-sweep partials (the per-lane sums in lred, the norm in lnrm) for the next
     // launch's decider: wave w reduces rows w, w + NW, ..., each in a fixed order
-    __syncthreads();  // every wave's lane sums are in LDS
     const size_t pstride = (size_t)kMaxSweepsPerLaunch * ntiles * 2;
     const int nupd = kLastOnly ? 1 : s_run;  // update rows (FP32: the last sweep's)
     const int nrows = nupd + (kNorm ? 1 : 0);
