@@ -78,9 +78,10 @@ typedef struct {
     int transport_fp32;    /* FP32_DC: 1 (default) = the transport and velocity stencils run in
                               FP32, each tap entering as an FP64 difference from the point's own
                               value (so their rounding is relative to the derivative, and the
-                              refined trajectory does not drift from the FP64 one: about 2e-8
-                              relative over 600 steps at 256^2, Re 1000); the state update
-                              stays FP64.  0 = FP64 stencils (1e-10 there).  Ignored by FP64  */
+                              refined trajectory does not drift from the FP64 one: about 2e-7
+                              relative over 600 steps at 256^2, Re 1000, mostly the velocities'
+                              own rounding); the state update stays FP64.  0 = FP64 stencils
+                              (1e-10 there).  Ignored by FP64                                 */
     double blowup_speed;   /* > 0: a step after which some |u| or |v| exceeds this (or is NaN)
                               ends with CNV_DIVERGED (the explicit scheme has gone unstable);
                               0 = off, as in the reference.  In a batch the cavity is dropped
