@@ -432,8 +432,9 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) 
 // Launch index L only matters through L mod 2 (state slots, partial sums) and L mod 3 (FP64
 // iterate planes), and every solve's state lives in the device slots, so a captured chunk of M
 // (a multiple of 6) SOR launches with L = 0..M-1 is replayed until the solve ends: each replay
-// continues the previous one's rotation, and after whole chunks the final slot is 1.  Two chunk sizes: launches_per_poll for the predicted length of the solve
-// and kTailChunk for its end, so that the no-op launches queued after convergence stay few.
+// continues the previous one's rotation, and after whole chunks the final slot is 1.  Two chunk
+// sizes: launches_per_poll for the predicted length of the solve and kTailChunk for its end, so
+// that the no-op launches queued after convergence stay few.
 static int capture(cnv_ctx* c, hipGraphExec_t* out, int (*body)(cnv_ctx*, int), int arg) {
     hipGraph_t g = nullptr;
     HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

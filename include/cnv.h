@@ -62,12 +62,12 @@ typedef struct {
     int tile_waves;        /* wave64s per SOR workgroup (compiled shapes only)               */
     int wave_rows;         /* grid rows held in registers per wave (compiled shapes only)    */
     int launches_per_poll; /* SOR launches queued between host polls of the convergence flag
-                              (0: per solve from the previous solve's length, 64 per captured
+                              (0: per solve from the previous solve's length, 66 per captured
                               chunk with use_graph, 2 for a batch)                          */
     int diagnostics;       /* 1 = continuity max/min every step (src/main.c:387-408)        */
     int use_graph;         /* 1 = each step replays captured HIP graphs (the pre-solve
                               launches, chunks of launches_per_poll SOR launches and tail
-                              chunks of 4, the post-solve launches; FP32_DC also the residual
+                              chunks of 6, the post-solve launches; FP32_DC also the residual
                               and correction of each outer iteration, with the sweep budget
                               kept on the device); results identical to the eager launches   */
     double inner_beta;     /* FP32_DC: relaxation of the inner FP32 SOR solves (0 = beta)     */
