@@ -251,7 +251,8 @@ def adjoint_leg(cnv, steps, K):
     """The adjoint workload: 256^2 cavity, Re 400, lid U = 1, dt = 20*2^-16, initial vorticity a sum
     of 32 Gaussians (seed 1); gradient of the final kinetic energy w.r.t. the initial vorticity and
     the lid velocity, checkpoints every K steps; checked by central differences (eps = 1e-6) along
-    one random direction and in U."""
+    one random direction and in U.  Solves start from 0; then once more with warm-started solves,
+    compared with the first gradient."""
     n, Re, U, dt = 256, 400.0, 1.0, 20 * 2.0 ** -16
     p = cnv.default_params(n, Re=Re, dt=dt, beta=cnv.reference_beta(n), itmax=50_000_000, diagnostics=0)
     w0 = cnv.gaussian_vorticity(n, count=32, seed=1)
@@ -269,6 +270,20 @@ def adjoint_leg(cnv, steps, K):
         d = np.random.default_rng(11).standard_normal((n, n))
         fd = (adj.forward(w0 + eps * d, U, steps) - adj.forward(w0 - eps * d, U, steps)) / (2 * eps)
         fdU = (adj.forward(w0, U + eps, steps) - adj.forward(w0, U - eps, steps)) / (2 * eps)
+    # the same with warm-started solves (forward from the previous step's psi, adjoint from the
+    # previous reverse step's solution)
+    p.warm_start = 1
+    with cnv.Adjoint(p, checkpoint_every=K, rtol_forward=1e-14, rtol_adjoint=1e-14) as adj:
+        t0 = time.perf_counter()
+        Ew = adj.forward(w0, U, steps)
+        tw_fwd = time.perf_counter() - t0
+        fws, _ = adj.sweeps()
+        t0 = time.perf_counter()
+        gw, dUw = adj.backward()
+        tw_bwd = time.perf_counter() - t0
+        _, bws = adj.sweeps()
+    print("bench: adjoint warm start forward %.2f s, backward %.2f s" % (tw_fwd, tw_bwd), file=sys.stderr,
+          flush=True)
     walls = np.concatenate([g[0, :], g[-1, :], g[:, 0], g[:, -1]])
     return {
         "workload": "cavity 256x256 Re400 FP64, 2000 steps, w0 = 32 Gaussians (seed 1), E = final kinetic "
@@ -280,6 +295,13 @@ def adjoint_leg(cnv, steps, K):
         "fd_rel_err_direction": abs(float(np.sum(g * d)) - fd) / abs(fd),
         "fd_rel_err_lid": abs(dU - fdU) / abs(fdU),
         "wall_gradient_max_abs": float(np.max(np.abs(walls))),
+        "warm_start": {
+            "forward_s": tw_fwd, "backward_s": tw_bwd,
+            "sweeps_forward": fws, "sweeps_backward_adjoint_solves": bws,
+            "energy_rel_diff": abs(Ew - E) / abs(E),
+            "gradient_max_rel_diff": float(np.max(np.abs(gw - g)) / np.max(np.abs(g))),
+            "dE_dU_rel_diff": abs(dUw - dU) / abs(dU),
+        },
     }
 
 

@@ -28,6 +28,11 @@
 // Checkpoints: the forward pass stores s_t every K steps in device memory; the backward pass
 // recomputes each K-step segment from its checkpoint (storing its states) and runs the reverse
 // steps through it, last segment first.
+//
+// Warm start (params warm_start): each forward solve starts from the previous step's psi and each
+// adjoint solve from the previous reverse step's solution, instead of from 0.  The checkpoints
+// then also hold the psi entering their step, and the adjoint solution is carried across
+// segments, so the gradient still does not depend on the checkpoint interval.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -212,6 +217,12 @@ struct cnv_adjoint {
     double *wt = nullptr, *src = nullptr, *dU = nullptr;
     double *uT = nullptr, *vT = nullptr;  // the last forward run's final velocities (the seed)
     int k_fwd = 0, k_adj = 0;  // queue-ahead predictions of the two kinds of solve
+    // warm start (params warm_start): each forward solve starts from the previous step's psi and
+    // each adjoint solve from the previous reverse step's solution; checkpoints then also hold
+    // the psi that enters their step
+    bool warm = false;
+    double* d_ckpsi = nullptr;  // [nckpt][cells]
+    double* zk = nullptr;       // the last adjoint solve's solution
     long long sweeps_fwd = 0, sweeps_adj = 0;
     char err[256] = "";
 };
@@ -236,6 +247,11 @@ static int afail(cnv_adjoint* a, int code, const char* fmt, ...) {
         hipError_t e_ = hipGetLastError();                                                            \
         if (e_ != hipSuccess) return afail((a), CNV_ERR_HIP, "adjoint launch failed: %s", hipGetErrorString(e_)); \
     } while (0)
+// a launch helper of the step kernels (nonzero when the launch failed)
+#define ALAUNCH_OF(a, x)                                                                              \
+    do {                                                                                              \
+        if ((x) != 0) return afail((a), CNV_ERR_HIP, "adjoint launch failed: %s", #x);               \
+    } while (0)
 #define ACTX(a, x)                                                                                    \
     do {                                                                                              \
         const int rc_ = (x);                                                                          \
@@ -246,9 +262,10 @@ static dim3 cell_grid(int n) { return dim3((n + 63) / 64, (n + 3) / 4); }
 
 // Poisson solve on the context's grid with the relative stopping rule: psi[parity] <- A^-1(...)
 // for the source src read as f = -src.  Returns the sweeps taken.
+// With warm start the solve starts from psi[0] as the caller left it, else from 0.
 static int solve_rel(cnv_adjoint* a, const double* src, double rtol, int* kpred, int* sweeps) {
     cnv_ctx* c = a->c;
-    ACTX(a, poisson_init_launch(c->psi[0], c->ctl, nullptr, c->n, c->ld, 0.0, c->p.itmax, c->stream));
+    ALAUNCH_OF(a, poisson_init_launch(c->psi[0], c->ctl, nullptr, c->n, c->ld, 0.0, c->p.itmax, c->stream, !a->warm));
     hipLaunchKernelGGL(adj_slots_kernel, dim3(1), dim3(64), 0, c->stream, c->ctl, 1e-300, rtol, c->p.itmax);
     ALAUNCH(a);
     SorArgs<double> sa = sor_args64(c, src, 1);
@@ -265,20 +282,33 @@ static int solve_rel(cnv_adjoint* a, const double* src, double rtol, int* kpred,
     return CNV_OK;
 }
 
-// one forward step on the context state (w[wcur], u, v), Poisson solve on the relative rule
-static int fwd_step(cnv_adjoint* a) {
+// one forward step on the context state (w[wcur], u, v), Poisson solve on the relative rule;
+// with warm start its first iterate is the last solve's final one, unless the caller has put
+// the step's own in psi[0] (start_given)
+static int fwd_step(cnv_adjoint* a, bool start_given) {
     cnv_ctx* c = a->c;
     const int n = c->n, ld = c->ld;
+    if (a->warm && !start_given)
+        ALAUNCH_OF(a, warm_start_launch(c->psi[0], planes_of(c), c->ctl + c->fin_slot, n, ld, c->stream));
     double* w_in = c->w[c->wcur];
     double* w_out = c->w[c->wcur ^ 1];
-    ACTX(a, wall_vorticity_launch(w_in, c->u, c->v, n, ld, c->st, c->bc, c->stream));
-    ACTX(a, transport_fp64_launch(w_in, w_out, c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re, c->p.dt, c->stream));
+    ALAUNCH_OF(a, wall_vorticity_launch(w_in, c->u, c->v, n, ld, c->st, c->bc, c->stream));
+    ALAUNCH_OF(a, transport_fp64_launch(w_in, w_out, c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re, c->p.dt, c->stream));
     c->wcur ^= 1;
     int k = 0;
     int rc = solve_rel(a, w_out, a->rtol_fwd, &a->k_fwd, &k);
     if (rc) return rc;
     a->sweeps_fwd += k;
-    ACTX(a, velocity_launch(planes_of(c), c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
+    ALAUNCH_OF(a, velocity_launch(planes_of(c), c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
+    return CNV_OK;
+}
+
+// warm start: psi[0] <- the psi entering checkpointed step idx (save: from psi[0])
+static int ckpt_psi(cnv_adjoint* a, int idx, bool save) {
+    cnv_ctx* c = a->c;
+    double* ck = a->d_ckpsi + (size_t)idx * a->cells;
+    const size_t bytes = sizeof(double) * a->cells;
+    AHIP(a, hipMemcpyAsync(save ? ck : c->psi[0], save ? c->psi[0] : ck, bytes, hipMemcpyDeviceToDevice, c->stream));
     return CNV_OK;
 }
 
@@ -311,9 +341,12 @@ static int rev_step(cnv_adjoint* a, int idx) {
     hipLaunchKernelGGL(adj_vel_src_kernel, g, blk, 0, c->stream, a->lu, a->lv, a->src, n, ld, a->ts, c->bc, c3);
     ALAUNCH(a);
     int k = 0;
+    if (a->warm)
+        AHIP(a, hipMemcpyAsync(c->psi[0], a->zk, sizeof(double) * a->cells, hipMemcpyDeviceToDevice, c->stream));
     int rc = solve_rel(a, a->src, a->rtol_adj, &a->k_adj, &k);
     if (rc) return rc;
     a->sweeps_adj += k;
+    if (a->warm) ALAUNCH_OF(a, warm_start_launch(a->zk, planes_of(c), c->ctl + c->fin_slot, n, ld, c->stream));
     hipLaunchKernelGGL(adj_accum_kernel, g, blk, 0, c->stream, a->lw, planes_of(c), c->ctl + c->fin_slot, n,
                        ld, c3);
     ALAUNCH(a);
@@ -321,7 +354,7 @@ static int rev_step(cnv_adjoint* a, int idx) {
     const double* us = slot3(a->d_seg, idx, 1, a->cells);
     const double* vs = slot3(a->d_seg, idx, 2, a->cells);
     AHIP(a, hipMemcpyAsync(a->wt, ws, sizeof(double) * a->cells, hipMemcpyDeviceToDevice, c->stream));
-    ACTX(a, wall_vorticity_launch(a->wt, us, vs, n, ld, c->st, c->bc, c->stream));
+    ALAUNCH_OF(a, wall_vorticity_launch(a->wt, us, vs, n, ld, c->st, c->bc, c->stream));
     hipLaunchKernelGGL(adj_transport_kernel, g, blk, 0, c->stream, a->lw, a->wt, us, vs, a->lwt, a->lub, a->lvb, n, ld,
                        c->st, a->ts, c->bc, 1. / c->p.Re, c->p.dt);
     ALAUNCH(a);
@@ -379,8 +412,8 @@ void cnv_adj_destroy(cnv_adjoint* a) {
         (void)hipStreamSynchronize(a->c->stream);
         cnv_destroy(a->c);
     }
-    void* dev[] = {a->d_tstart, a->d_ttab, a->d_ckpt, a->d_seg, a->lw, a->lu, a->lv, a->lwt, a->lub, a->lvb,
-                   a->wt, a->src, a->dU, a->uT, a->vT};
+    void* dev[] = {a->d_tstart, a->d_ttab, a->d_ckpt, a->d_ckpsi, a->d_seg, a->lw, a->lu, a->lv, a->lwt, a->lub,
+                   a->lvb, a->wt, a->src, a->dU, a->uT, a->vT, a->zk};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     delete a;
@@ -409,6 +442,10 @@ int cnv_adj_create(const cnv_params* p, int checkpoint_every, double rtol_forwar
         double** bufs[] = {&a->lw, &a->lu, &a->lv, &a->lwt, &a->lub, &a->lvb, &a->wt, &a->src, &a->uT, &a->vT};
         for (double** b : bufs) AHIP(a, hipMalloc(b, bytes));
         AHIP(a, hipMalloc(&a->dU, sizeof(double)));
+        if (p->warm_start) {
+            a->warm = true;
+            AHIP(a, hipMalloc(&a->zk, bytes));
+        }
         return build_transposed(a);
     };
     const int rc = setup();
@@ -432,8 +469,10 @@ int cnv_adj_forward(cnv_adjoint* a, const double* w0, double U, int nsteps, doub
     const int nck = (nsteps + a->K - 1) / a->K;
     if (nck > a->nckpt_cap) {
         if (a->d_ckpt) AHIP(a, hipFree(a->d_ckpt));
-        a->d_ckpt = nullptr;
+        if (a->d_ckpsi) AHIP(a, hipFree(a->d_ckpsi));
+        a->d_ckpt = a->d_ckpsi = nullptr;
         AHIP(a, hipMalloc(&a->d_ckpt, bytes * 3 * nck));
+        if (a->warm) AHIP(a, hipMalloc(&a->d_ckpsi, bytes * nck));
         a->nckpt_cap = nck;
     }
     // initial state: w0; u = ui, v = vi inside, 0 on the edges (src/main.c:178-221); psi = 0
@@ -451,12 +490,18 @@ int cnv_adj_forward(cnv_adjoint* a, const double* w0, double U, int nsteps, doub
     a->U = U;
     a->T = nsteps;
     a->sweeps_fwd = 0;
+    if (a->warm) AHIP(a, hipMemsetAsync(c->psi[0], 0, bytes, c->stream));  // step 0 starts from 0
     for (int t = 0; t < nsteps; ++t) {
-        if (t % a->K == 0) {
+        const bool ck = t % a->K == 0;
+        if (ck) {
             int rc = save_state(a, a->d_ckpt, t / a->K);
             if (rc) return rc;
+            // the psi entering step t: the last solve's final iterate (0 before step 0)
+            if (a->warm && t > 0)
+                ALAUNCH_OF(a, warm_start_launch(c->psi[0], planes_of(c), c->ctl + c->fin_slot, n, c->ld, c->stream));
+            if (a->warm && (rc = ckpt_psi(a, t / a->K, true))) return rc;
         }
-        int rc = fwd_step(a);
+        int rc = fwd_step(a, ck);
         if (rc) return rc;
     }
     // the backward pass seeds from these: its recompute moves the context's own state
@@ -486,15 +531,17 @@ int cnv_adj_backward(cnv_adjoint* a, double* dE_dw0, double* dE_dU) {
     ALAUNCH(a);
     hipLaunchKernelGGL(adj_zero_kernel, dim3(1), dim3(kBlock), 0, c->stream, a->dU, 1LL);
     ALAUNCH(a);
+    if (a->warm) AHIP(a, hipMemsetAsync(a->zk, 0, sizeof(double) * a->cells, c->stream));  // the last step's z starts from 0
     a->sweeps_adj = 0;
     const int nseg = (a->T + a->K - 1) / a->K;
     for (int sgi = nseg - 1; sgi >= 0; --sgi) {
         const int t0 = sgi * a->K, t1 = std::min(a->T, t0 + a->K);
         int rc = load_state(a, a->d_ckpt, sgi);
+        if (!rc && a->warm) rc = ckpt_psi(a, sgi, false);
         if (rc) return rc;
         for (int t = t0; t < t1; ++t) {  // recompute the segment, storing each step's input state
             rc = save_state(a, a->d_seg, t - t0);
-            if (!rc) rc = fwd_step(a);
+            if (!rc) rc = fwd_step(a, t == t0);
             if (rc) return rc;
         }
         for (int t = t1 - 1; t >= t0; --t) {

@@ -74,7 +74,9 @@ typedef struct {
     int warm_start;        /* 1 = each Poisson solve starts from the previous step's psi
                               instead of 0 (src/poisson.c:229 starts from 0): fewer sweeps,
                               results that differ within the tolerance; off for parity.
-                              One-cavity contexts only (a batch refuses it)                  */
+                              One-cavity contexts and the adjoint (whose adjoint solves then
+                              start from the previous reverse step's solution); a batch
+                              refuses it                                                     */
     int transport_fp32;    /* FP32_DC: 1 (default) = the transport and velocity stencils run in
                               FP32, each tap entering as an FP64 difference from the point's own
                               value (so their rounding is relative to the derivative, and the

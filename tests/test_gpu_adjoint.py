@@ -13,8 +13,9 @@ from adjoint_ref import CavityAdjoint
 pytestmark = pytest.mark.gpu
 
 
-def make(n, Re, dt, K):
-    p = cnv.default_params(n, Re=Re, dt=dt, beta=cnv.reference_beta(n), itmax=10_000_000, diagnostics=0)
+def make(n, Re, dt, K, warm_start=0):
+    p = cnv.default_params(n, Re=Re, dt=dt, beta=cnv.reference_beta(n), itmax=10_000_000, diagnostics=0,
+                           warm_start=warm_start)
     return cnv.Adjoint(p, checkpoint_every=K, rtol_forward=1e-14, rtol_adjoint=1e-14)
 
 
@@ -72,6 +73,39 @@ def test_gpu_adjoint_checkpoint_interval_does_not_change_the_gradient():
         assert E == out[0][0] and dU == out[0][2] and np.array_equal(g, out[0][1])
 
 
+def test_gpu_adjoint_warm_start():
+    # warm-started solves (forward from the previous step's psi, adjoint from the previous reverse
+    # step's solution) take fewer sweeps and give the gradient of the CPU adjoint within the same
+    # bounds; the checkpoints carry the psi entering their step and the adjoint solution is carried
+    # across segments, so the checkpoint interval and a second backward call change nothing
+    n, steps, U, Re, dt = 24, 9, 1.1, 400.0, 4e-3
+    w0 = cnv.gaussian_vorticity(n, count=8, seed=2)
+    E_ref, g_ref, dU_ref = CavityAdjoint(n, Re, dt).gradient(w0, U, steps)
+    out = []
+    for K in (1, 4, 100):
+        print("adjoint warm start K=%d" % K, flush=True)
+        with make(n, Re, dt, K, warm_start=1) as adj:
+            E = adj.forward(w0, U, steps)
+            g, dU = adj.backward()
+            warm = adj.sweeps()  # forward + recompute, adjoint
+            g2, dU2 = adj.backward()
+        assert dU2 == dU and np.array_equal(g2, g)
+        out.append((E, g, dU))
+    for E, g, dU in out[1:]:
+        assert E == out[0][0] and dU == out[0][2] and np.array_equal(g, out[0][1])
+    E, g, dU = out[0]
+    assert abs(E - E_ref) <= 1e-11 * abs(E_ref)
+    assert np.max(np.abs(g - g_ref)) <= 1e-8 * np.max(np.abs(g_ref))
+    assert abs(dU - dU_ref) <= 1e-8 * abs(dU_ref)
+    assert np.all(g[walls(n)] == 0.0)
+    with make(n, Re, dt, 100) as adj:
+        adj.forward(w0, U, steps)
+        adj.backward()
+        cold = adj.sweeps()
+    print("sweeps warm", warm, "cold", cold, flush=True)
+    assert warm[0] < cold[0] and warm[1] < cold[1]
+
+
 def test_gpu_adjoint_through_torch_autograd():
     torch = pytest.importorskip("torch")
     n, steps, Re, dt = 16, 4, 300.0, 4e-3
@@ -114,7 +148,7 @@ def test_gpu_adjoint_workload_full_size():
     # the adjoint workload itself: 256^2, Re 400, dt = 20*2^-16, 2000 steps, w0 = the committed
     # 32 Gaussians (seed 1), lid U = 1, checkpoints every 100 steps; central differences
     # (eps = 1e-6) along one random direction and in U within 1e-5, and the same gradient with
-    # checkpoints every 50 steps
+    # checkpoints every 50 steps, and with warm-started solves within 1e-10
     n, steps, U, Re, dt = 256, 2000, 1.0, 400.0, 20 * 2.0 ** -16
     w0 = cnv.gaussian_vorticity(n)
     d = np.random.default_rng(11).standard_normal((n, n))
@@ -136,3 +170,10 @@ def test_gpu_adjoint_workload_full_size():
         g50, dU50 = adj.backward()
     print("adjoint workload: K=50 done", flush=True)
     assert dU50 == dU and np.array_equal(g50, g)
+    # warm-started solves: the same gradient within what the 1e-14 relative rule leaves
+    with make(n, Re, dt, K=100, warm_start=1) as adj:
+        Ew = adj.forward(w0, U, steps)
+        gw, dUw = adj.backward()
+    print("adjoint workload: warm start", Ew, dUw, flush=True)
+    assert abs(Ew - E) <= 1e-11 * abs(E)
+    assert np.max(np.abs(gw - g)) <= 1e-10 * np.max(np.abs(g)) and abs(dUw - dU) <= 1e-10 * abs(dU)
