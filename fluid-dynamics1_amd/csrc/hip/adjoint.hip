@@ -27,7 +27,8 @@
 //
 // Checkpoints: the forward pass stores s_t every K steps in device memory; the backward pass
 // recomputes each K-step segment from its checkpoint (storing its states) and runs the reverse
-// steps through it, last segment first.
+// steps through it, last segment first; the recompute of the segment before runs at the same time,
+// on a second context and stream driven by a second host thread (cnv_adj_backward).
 //
 // Warm start (params warm_start): each forward solve starts from the previous step's psi and each
 // adjoint solve from the previous reverse step's solution, instead of from 0.  The checkpoints
@@ -41,6 +42,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "cnv.h"
@@ -199,7 +203,8 @@ thread_local char g_aerr[256] = "";  // last error of this thread
 }  // namespace
 
 struct cnv_adjoint {
-    cnv_ctx* c = nullptr;
+    cnv_ctx* c = nullptr;   // forward runs and the backward pass's recompute
+    cnv_ctx* cz = nullptr;  // the reverse steps (their solves and kernels), on a stream of its own
     int K = 100;
     double rtol_fwd = 1e-14, rtol_adj = 1e-14;
     int T = 0;  // steps of the last forward run
@@ -212,7 +217,9 @@ struct cnv_adjoint {
     // storage
     double* d_ckpt = nullptr;  // [nckpt][3][cells]
     int nckpt_cap = 0;
-    double* d_seg = nullptr;   // [K][3][cells]
+    double* d_seg[2] = {nullptr, nullptr};  // [K][3][cells] each: segment s in d_seg[s & 1]
+    hipEvent_t ev_ready[2] = {nullptr, nullptr};  // d_seg[b] recomputed (recorded on c's stream)
+    hipEvent_t ev_free[2] = {nullptr, nullptr};   // d_seg[b] read by the reverse steps (cz's stream)
     double *lw = nullptr, *lu = nullptr, *lv = nullptr, *lwt = nullptr, *lub = nullptr, *lvb = nullptr;
     double *wt = nullptr, *src = nullptr, *dU = nullptr;
     double *uT = nullptr, *vT = nullptr;  // the last forward run's final velocities (the seed)
@@ -223,9 +230,11 @@ struct cnv_adjoint {
     bool warm = false;
     double* d_ckpsi = nullptr;  // [nckpt][cells]
     double* zk = nullptr;       // the last adjoint solve's solution
-    long long sweeps_fwd = 0, sweeps_adj = 0;
+    long long sweeps_fwd = 0, sweeps_adj = 0, sweeps_rec = 0;  // forward run; backward: adjoint, recompute
     char err[256] = "";
 };
+
+std::mutex g_aerr_mu;  // a->err is written by both threads of a backward pass
 
 static int afail(cnv_adjoint* a, int code, const char* fmt, ...) {
     char buf[256];
@@ -233,7 +242,10 @@ static int afail(cnv_adjoint* a, int code, const char* fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(buf, sizeof(buf), fmt, ap);
     va_end(ap);
-    if (a) snprintf(a->err, sizeof(a->err), "%s", buf);
+    if (a) {
+        std::lock_guard<std::mutex> l(g_aerr_mu);
+        snprintf(a->err, sizeof(a->err), "%s", buf);
+    }
     snprintf(g_aerr, sizeof(g_aerr), "%s", buf);
     return code;
 }
@@ -263,8 +275,7 @@ static dim3 cell_grid(int n) { return dim3((n + 63) / 64, (n + 3) / 4); }
 // Poisson solve on the context's grid with the relative stopping rule: psi[parity] <- A^-1(...)
 // for the source src read as f = -src.  Returns the sweeps taken.
 // With warm start the solve starts from psi[0] as the caller left it, else from 0.
-static int solve_rel(cnv_adjoint* a, const double* src, double rtol, int* kpred, int* sweeps) {
-    cnv_ctx* c = a->c;
+static int solve_rel(cnv_adjoint* a, cnv_ctx* c, const double* src, double rtol, int* kpred, int* sweeps) {
     ALAUNCH_OF(a, poisson_init_launch(c->psi[0], c->ctl, nullptr, c->n, c->ld, 0.0, c->p.itmax, c->stream, !a->warm));
     hipLaunchKernelGGL(adj_slots_kernel, dim3(1), dim3(64), 0, c->stream, c->ctl, 1e-300, rtol, c->p.itmax);
     ALAUNCH(a);
@@ -285,7 +296,7 @@ static int solve_rel(cnv_adjoint* a, const double* src, double rtol, int* kpred,
 // one forward step on the context state (w[wcur], u, v), Poisson solve on the relative rule;
 // with warm start its first iterate is the last solve's final one, unless the caller has put
 // the step's own in psi[0] (start_given)
-static int fwd_step(cnv_adjoint* a, bool start_given) {
+static int fwd_step(cnv_adjoint* a, bool start_given, long long* sweep_count) {
     cnv_ctx* c = a->c;
     const int n = c->n, ld = c->ld;
     if (a->warm && !start_given)
@@ -296,9 +307,9 @@ static int fwd_step(cnv_adjoint* a, bool start_given) {
     ALAUNCH_OF(a, transport_fp64_launch(w_in, w_out, c->u, c->v, n, ld, c->st, c->bc, 1. / c->p.Re, c->p.dt, c->stream));
     c->wcur ^= 1;
     int k = 0;
-    int rc = solve_rel(a, w_out, a->rtol_fwd, &a->k_fwd, &k);
+    int rc = solve_rel(a, c, w_out, a->rtol_fwd, &a->k_fwd, &k);
     if (rc) return rc;
-    a->sweeps_fwd += k;
+    *sweep_count += k;
     ALAUNCH_OF(a, velocity_launch(planes_of(c), c->ctl + c->fin_slot, c->u, c->v, n, ld, c->st, c->stream));
     return CNV_OK;
 }
@@ -331,9 +342,10 @@ static int load_state(cnv_adjoint* a, double* base, int idx) {
     return CNV_OK;
 }
 
-// reverse of one step whose input state is stored at seg slot idx; (lw, lu, lv) in/out
-static int rev_step(cnv_adjoint* a, int idx) {
-    cnv_ctx* c = a->c;
+// reverse of one step whose input state is stored at slot idx of seg; (lw, lu, lv) in/out; on the
+// reverse steps' context cz
+static int rev_step(cnv_adjoint* a, double* seg, int idx) {
+    cnv_ctx* c = a->cz;
     const int n = c->n, ld = c->ld;
     const double dx = c->p.dx, dy = c->p.dy;
     const double c3 = dx * dx * dy * dy;
@@ -343,16 +355,16 @@ static int rev_step(cnv_adjoint* a, int idx) {
     int k = 0;
     if (a->warm)
         AHIP(a, hipMemcpyAsync(c->psi[0], a->zk, sizeof(double) * a->cells, hipMemcpyDeviceToDevice, c->stream));
-    int rc = solve_rel(a, a->src, a->rtol_adj, &a->k_adj, &k);
+    int rc = solve_rel(a, c, a->src, a->rtol_adj, &a->k_adj, &k);
     if (rc) return rc;
     a->sweeps_adj += k;
     if (a->warm) ALAUNCH_OF(a, warm_start_launch(a->zk, planes_of(c), c->ctl + c->fin_slot, n, ld, c->stream));
     hipLaunchKernelGGL(adj_accum_kernel, g, blk, 0, c->stream, a->lw, planes_of(c), c->ctl + c->fin_slot, n,
                        ld, c3);
     ALAUNCH(a);
-    const double* ws = slot3(a->d_seg, idx, 0, a->cells);
-    const double* us = slot3(a->d_seg, idx, 1, a->cells);
-    const double* vs = slot3(a->d_seg, idx, 2, a->cells);
+    const double* ws = slot3(seg, idx, 0, a->cells);
+    const double* us = slot3(seg, idx, 1, a->cells);
+    const double* vs = slot3(seg, idx, 2, a->cells);
     AHIP(a, hipMemcpyAsync(a->wt, ws, sizeof(double) * a->cells, hipMemcpyDeviceToDevice, c->stream));
     ALAUNCH_OF(a, wall_vorticity_launch(a->wt, us, vs, n, ld, c->st, c->bc, c->stream));
     hipLaunchKernelGGL(adj_transport_kernel, g, blk, 0, c->stream, a->lw, a->wt, us, vs, a->lwt, a->lub, a->lvb, n, ld,
@@ -408,12 +420,15 @@ const char* cnv_adj_last_error(const cnv_adjoint* a) { return a ? a->err : g_aer
 
 void cnv_adj_destroy(cnv_adjoint* a) {
     if (!a) return;
-    if (a->c) {
-        (void)hipStreamSynchronize(a->c->stream);
-        cnv_destroy(a->c);
-    }
-    void* dev[] = {a->d_tstart, a->d_ttab, a->d_ckpt, a->d_ckpsi, a->d_seg, a->lw, a->lu, a->lv, a->lwt, a->lub,
-                   a->lvb, a->wt, a->src, a->dU, a->uT, a->vT, a->zk};
+    for (cnv_ctx* c : {a->c, a->cz})
+        if (c) {
+            (void)hipStreamSynchronize(c->stream);
+            cnv_destroy(c);
+        }
+    for (hipEvent_t e : {a->ev_ready[0], a->ev_ready[1], a->ev_free[0], a->ev_free[1]})
+        if (e) (void)hipEventDestroy(e);
+    void* dev[] = {a->d_tstart, a->d_ttab, a->d_ckpt, a->d_ckpsi, a->d_seg[0], a->d_seg[1], a->lw, a->lu, a->lv,
+                   a->lwt, a->lub, a->lvb, a->wt, a->src, a->dU, a->uT, a->vT, a->zk};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     delete a;
@@ -434,11 +449,16 @@ int cnv_adj_create(const cnv_params* p, int checkpoint_every, double rtol_forwar
     auto setup = [&]() -> int {
         cnv_params q = *p;
         q.diagnostics = 0;
-        const int rc = cnv_create(&q, &a->c);
+        int rc = cnv_create(&q, &a->c);
+        if (!rc) rc = cnv_create(&q, &a->cz);
         if (rc) return afail(a, rc, "%s", cnv_last_error(nullptr));
         a->cells = a->c->cells;
         const size_t bytes = sizeof(double) * a->cells;
-        AHIP(a, hipMalloc(&a->d_seg, bytes * 3 * a->K));
+        for (int b = 0; b < 2; ++b) {
+            AHIP(a, hipMalloc(&a->d_seg[b], bytes * 3 * a->K));
+            AHIP(a, hipEventCreateWithFlags(&a->ev_ready[b], hipEventDisableTiming));
+            AHIP(a, hipEventCreateWithFlags(&a->ev_free[b], hipEventDisableTiming));
+        }
         double** bufs[] = {&a->lw, &a->lu, &a->lv, &a->lwt, &a->lub, &a->lvb, &a->wt, &a->src, &a->uT, &a->vT};
         for (double** b : bufs) AHIP(a, hipMalloc(b, bytes));
         AHIP(a, hipMalloc(&a->dU, sizeof(double)));
@@ -501,7 +521,7 @@ int cnv_adj_forward(cnv_adjoint* a, const double* w0, double U, int nsteps, doub
                 ALAUNCH_OF(a, warm_start_launch(c->psi[0], planes_of(c), c->ctl + c->fin_slot, n, c->ld, c->stream));
             if (a->warm && (rc = ckpt_psi(a, t / a->K, true))) return rc;
         }
-        int rc = fwd_step(a, ck);
+        int rc = fwd_step(a, ck, &a->sweeps_fwd);
         if (rc) return rc;
     }
     // the backward pass seeds from these: its recompute moves the context's own state
@@ -517,43 +537,125 @@ int cnv_adj_forward(cnv_adjoint* a, const double* w0, double U, int nsteps, doub
     return CNV_OK;
 }
 
+// recompute of segment s from its checkpoint on the context c, storing each step's input state
+// in d_seg[b]
+static int recompute(cnv_adjoint* a, int s, int b) {
+    const int t0 = s * a->K, t1 = std::min(a->T, t0 + a->K);
+    int rc = load_state(a, a->d_ckpt, s);
+    if (!rc && a->warm) rc = ckpt_psi(a, s, false);
+    for (int t = t0; t < t1 && !rc; ++t) {
+        rc = save_state(a, a->d_seg[b], t - t0);
+        if (!rc) rc = fwd_step(a, t == t0, &a->sweeps_rec);
+    }
+    return rc;
+}
+
+// The backward pass runs two pipelines at once.  A second thread recomputes the segments, last
+// first, on the context c; this thread runs each segment's reverse steps on the context cz as
+// soon as its recompute is done.  Segment s goes to d_seg[s & 1], so the recompute of segment s - 1
+// overlaps the reverse steps of segment s; the recompute of s - 2 waits until those reverse steps
+// have read the buffer.  The two streams are ordered by events (ev_ready[b] after a recompute,
+// ev_free[b] after its reverse steps), and each thread queues its wait only after the host has seen
+// the other thread record the event it waits for.  Every kernel sees the same inputs as it would
+// one pipeline after the other, so the gradient is the same bit for bit.
 int cnv_adj_backward(cnv_adjoint* a, double* dE_dw0, double* dE_dU) {
     if (!a || a->T < 1) return afail(a, CNV_ERR_ARG, "adjoint backward: run cnv_adj_forward first");
     cnv_ctx* c = a->c;
+    cnv_ctx* z = a->cz;
     AHIP(a, hipSetDevice(c->p.device));
     const int n = c->n, ld = c->ld;
     const dim3 g = cell_grid(n), blk(kBlock);
+    AHIP(a, hipStreamSynchronize(c->stream));
+    AHIP(a, hipStreamSynchronize(z->stream));
+    z->bc = c->bc;  // the wall velocities of the last forward run (its lid velocity U)
     // seed from the last forward run's final state, kept apart from the context's state, which
     // each backward pass's recompute moves (to step min(T, K) by its end): a second backward
     // call after one forward run returns the same gradient
-    hipLaunchKernelGGL(adj_seed_kernel, g, blk, 0, c->stream, a->uT, a->vT, a->lw, a->lu, a->lv, n, ld,
+    hipLaunchKernelGGL(adj_seed_kernel, g, blk, 0, z->stream, a->uT, a->vT, a->lw, a->lu, a->lv, n, ld,
                        c->p.dx * c->p.dy);
     ALAUNCH(a);
-    hipLaunchKernelGGL(adj_zero_kernel, dim3(1), dim3(kBlock), 0, c->stream, a->dU, 1LL);
+    hipLaunchKernelGGL(adj_zero_kernel, dim3(1), dim3(kBlock), 0, z->stream, a->dU, 1LL);
     ALAUNCH(a);
-    if (a->warm) AHIP(a, hipMemsetAsync(a->zk, 0, sizeof(double) * a->cells, c->stream));  // the last step's z starts from 0
-    a->sweeps_adj = 0;
+    if (a->warm) AHIP(a, hipMemsetAsync(a->zk, 0, sizeof(double) * a->cells, z->stream));  // the last step's z starts from 0
+    a->sweeps_adj = a->sweeps_rec = 0;
     const int nseg = (a->T + a->K - 1) / a->K;
-    for (int sgi = nseg - 1; sgi >= 0; --sgi) {
-        const int t0 = sgi * a->K, t1 = std::min(a->T, t0 + a->K);
-        int rc = load_state(a, a->d_ckpt, sgi);
-        if (!rc && a->warm) rc = ckpt_psi(a, sgi, false);
-        if (rc) return rc;
-        for (int t = t0; t < t1; ++t) {  // recompute the segment, storing each step's input state
-            rc = save_state(a, a->d_seg, t - t0);
-            if (!rc) rc = fwd_step(a, t == t0);
-            if (rc) return rc;
+
+    std::mutex m;
+    std::condition_variable cv;
+    int recomputed = nseg, reversed = nseg;  // lowest segment whose recompute / reverse steps are queued
+    bool stop = false;
+    auto post = [&](int& which, int sg) {
+        {
+            std::lock_guard<std::mutex> l(m);
+            which = sg;
         }
-        for (int t = t1 - 1; t >= t0; --t) {
-            rc = rev_step(a, t - t0);
-            if (rc) return rc;
+        cv.notify_all();
+    };
+    auto halt = [&]() {
+        {
+            std::lock_guard<std::mutex> l(m);
+            stop = true;
         }
+        cv.notify_all();
+    };
+    auto wait_until = [&](const int& which, int sg) {  // false: the other thread stopped
+        std::unique_lock<std::mutex> l(m);
+        cv.wait(l, [&] { return stop || which <= sg; });
+        return !stop;
+    };
+    int rc_rec = CNV_OK;
+    char err_rec[256] = "";
+    std::thread rec([&]() {
+        int rc = CNV_OK;
+        if (hipSetDevice(c->p.device) != hipSuccess) rc = afail(a, CNV_ERR_HIP, "hipSetDevice failed in the recompute");
+        for (int sg = nseg - 1; sg >= 0 && !rc; --sg) {
+            const int b = sg & 1;
+            if (sg + 2 < nseg) {  // d_seg[b] holds segment sg + 2 until its reverse steps have run
+                if (!wait_until(reversed, sg + 2)) break;
+                if (hipStreamWaitEvent(c->stream, a->ev_free[b], 0) != hipSuccess) {
+                    rc = afail(a, CNV_ERR_HIP, "hipStreamWaitEvent failed in the recompute");
+                    break;
+                }
+            }
+            rc = recompute(a, sg, b);
+            if (!rc && hipEventRecord(a->ev_ready[b], c->stream) != hipSuccess)
+                rc = afail(a, CNV_ERR_HIP, "hipEventRecord failed in the recompute");
+            if (!rc) post(recomputed, sg);
+        }
+        if (rc) {
+            {
+                std::lock_guard<std::mutex> l(g_aerr_mu);
+                snprintf(err_rec, sizeof(err_rec), "%s", a->err);
+            }
+            rc_rec = rc;
+            halt();
+        }
+    });
+    int rc = CNV_OK;
+    for (int sg = nseg - 1; sg >= 0 && !rc; --sg) {
+        const int b = sg & 1;
+        const int t0 = sg * a->K, t1 = std::min(a->T, t0 + a->K);
+        if (!wait_until(recomputed, sg)) break;
+        if (hipStreamWaitEvent(z->stream, a->ev_ready[b], 0) != hipSuccess)
+            rc = afail(a, CNV_ERR_HIP, "hipStreamWaitEvent failed in the reverse steps");
+        for (int t = t1 - 1; t >= t0 && !rc; --t) rc = rev_step(a, a->d_seg[b], t - t0);
+        if (!rc && hipEventRecord(a->ev_free[b], z->stream) != hipSuccess)
+            rc = afail(a, CNV_ERR_HIP, "hipEventRecord failed in the reverse steps");
+        if (rc) halt();
+        else post(reversed, sg);
     }
+    rec.join();
+    // nothing of this call stays in flight, whatever ended it
+    const hipError_t e1 = hipStreamSynchronize(c->stream), e2 = hipStreamSynchronize(z->stream);
+    if (!rc && rc_rec) rc = afail(a, rc_rec, "%s", err_rec);
+    if (rc) return rc;
+    if (e1 != hipSuccess || e2 != hipSuccess)
+        return afail(a, CNV_ERR_HIP, "backward pass: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
     const size_t bytes = sizeof(double) * a->cells;
-    if (dE_dw0) AHIP(a, hipMemcpyAsync(dE_dw0, a->lw, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (dE_dw0) AHIP(a, hipMemcpyAsync(dE_dw0, a->lw, bytes, hipMemcpyDeviceToHost, z->stream));
     double dU = 0.0;
-    AHIP(a, hipMemcpyAsync(&dU, a->dU, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    AHIP(a, hipStreamSynchronize(c->stream));
+    AHIP(a, hipMemcpyAsync(&dU, a->dU, sizeof(double), hipMemcpyDeviceToHost, z->stream));
+    AHIP(a, hipStreamSynchronize(z->stream));
     if (dE_dU) *dE_dU = dU;
     return CNV_OK;
 }
