@@ -264,10 +264,11 @@ static int afail(cnv_adjoint* a, int code, const char* fmt, ...) {
     do {                                                                                              \
         if ((x) != 0) return afail((a), CNV_ERR_HIP, "adjoint launch failed: %s", #x);               \
     } while (0)
-#define ACTX(a, x)                                                                                    \
+// a call on the context ctx that returns a status and leaves its message in ctx->err
+#define ACTX(a, ctx, x)                                                                               \
     do {                                                                                              \
         const int rc_ = (x);                                                                          \
-        if (rc_) return afail((a), rc_, "%s", (a)->c->err);                                          \
+        if (rc_) return afail((a), rc_, "%s", (ctx)->err);                                           \
     } while (0)
 
 static dim3 cell_grid(int n) { return dim3((n + 63) / 64, (n + 3) / 4); }
@@ -283,7 +284,7 @@ static int solve_rel(cnv_adjoint* a, cnv_ctx* c, const double* src, double rtol,
     sa.norm = 1;
     SorCtl fin{};
     c->k_prev = *kpred;
-    ACTX(a, run_sor<double>(c, sa, &fin));
+    ACTX(a, c, run_sor<double>(c, sa, &fin));
     if (fin.status == SOR_DIVERGED)
         return afail(a, CNV_DIVERGED, "adjoint-problem Poisson solve diverged (non-finite update) after %d sweeps", fin.k);
     if (fin.status == SOR_FAILED)

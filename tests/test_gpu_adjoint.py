@@ -106,6 +106,23 @@ def test_gpu_adjoint_warm_start():
     assert warm[0] < cold[0] and warm[1] < cold[1]
 
 
+def test_gpu_adjoint_backward_failure_ends_both_pipelines():
+    # an adjoint-problem solve that hits itmax (a loose forward rule converges well within it, the
+    # 1e-14 adjoint rule cannot) ends the backward pass with CNV_NOT_CONVERGED while the recompute
+    # thread is still running or waiting: the call returns the error instead of hanging, and a
+    # second call does the same
+    n, steps, U, Re, dt = 24, 6, 1.0, 300.0, 4e-3
+    p = cnv.default_params(n, Re=Re, dt=dt, beta=cnv.reference_beta(n), itmax=60, diagnostics=0)
+    w0 = cnv.gaussian_vorticity(n, count=6, seed=3)
+    with cnv.Adjoint(p, checkpoint_every=2, rtol_forward=1e-2, rtol_adjoint=1e-14) as adj:
+        adj.forward(w0, U, steps)
+        for _ in range(2):
+            with pytest.raises(cnv.CnvError) as ei:
+                adj.backward()
+            assert ei.value.code == cnv.NOT_CONVERGED
+            assert "adjoint-problem Poisson solve did not converge" in str(ei.value)
+
+
 def test_gpu_adjoint_through_torch_autograd():
     torch = pytest.importorskip("torch")
     n, steps, Re, dt = 16, 4, 300.0, 4e-3
