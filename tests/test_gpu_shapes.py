@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 # S, waves, rows per wave: the shapes the library compiles
 SHAPES = [(1, 4, 10), (2, 4, 10), (4, 8, 8), (6, 8, 8), (8, 8, 10), (4, 16, 4), (5, 16, 4), (6, 16, 4),
-          (5, 12, 6), (5, 8, 4), (4, 8, 4), (5, 4, 6)]
+          (5, 12, 6), (5, 8, 4), (5, 8, 8), (4, 8, 4), (5, 4, 6)]
 WALLS = dict(ui=0.1, vi=-0.05, u=(0.2, -0.1, 0.05, 1.2), v=(-0.3, 0.1, 0.25, -0.15))
 
 
