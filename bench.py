@@ -49,9 +49,9 @@ SIMDS_PER_CU = 4
 
 def workload_params(cnv, precision, args, **kw):
     # FP64 steps replay captured HIP graphs unless --eager; the refined form runs eager launches,
-    # which measured 1.1% faster (6% with the reference's inner relaxation): its step needs tens
-    # of graph launches (each outer iteration's residual, chunks and correction), each with its
-    # own submission cost
+    # as fast as replay at omega 1.95 and 3% faster with the reference's inner relaxation: its
+    # step needs tens of graph launches (each outer iteration's residual, chunks and correction),
+    # each with its own submission cost
     return cnv.default_params(N, Re=RE, dt=DT, order=6, beta=OMEGA, tol=TOL, itmax=50_000_000,
                               precision=precision, sweeps_per_launch=args.spl, tile_waves=args.tile_waves,
                               wave_rows=args.wave_rows, diagnostics=1,

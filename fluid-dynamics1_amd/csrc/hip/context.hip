@@ -539,16 +539,17 @@ static int build_graphs(cnv_ctx* c) {
     return capture(c, &c->g_post, body_post, 0);
 }
 
-// Replays chunk graphs until a polled slot is terminal.  With a prediction of P launches the
-// first big chunks cover about P - 2*kTailChunk launches and tail chunks follow, as many per
-// poll as make up the eager solve's chunk for that prediction (poll_chunk); without one, big
-// chunks one per poll.
+// Replays chunk graphs until a polled slot is terminal.  With a prediction of P launches, big
+// chunks and then tail chunks up to about P - 2*kTailChunk launches go before the first poll, and
+// tail chunks follow, as many per poll as make up the eager solve's chunk for that prediction
+// (poll_chunk); without one, big chunks one per poll.
 static int replay_solve(cnv_ctx* c, int src, int predicted_sweeps, SorCtl* final_ctl) {
     const int M = c->launches_per_poll, Mt = cnv_ctx::kTailChunk;
     const int S = c->shape.s;
     const int P = predicted_sweeps > 0 ? predicted_sweeps / S : 0;
     const long long cap = (long long)c->p.itmax / S + 4LL * M + 8;
     const int first = P > 0 ? std::max(0, (P - 2 * Mt) / M) : 1;  // big chunks before the first poll
+    const int extra = P > 0 ? std::max(0, (P - 2 * Mt - first * M) / Mt) : 0;  // and tail chunks
     const int tails = P > 0 ? std::max(1, (poll_chunk(c, P) + Mt / 2) / Mt) : 0;  // tail chunks per poll
     long long queued = 0;
     int slot = 0, round = 0;
@@ -556,9 +557,10 @@ static int replay_solve(cnv_ctx* c, int src, int predicted_sweeps, SorCtl* final
     int npend = 0;
     c->fin_slot = 1;
     while (true) {
-        if (round == 0 && first > 0) {
+        if (round == 0 && first + extra > 0) {
             for (int k = 0; k < first; ++k) HIPC(c, hipGraphLaunch(c->g_chunk[src], c->stream));
-            queued += (long long)first * M;
+            for (int k = 0; k < extra; ++k) HIPC(c, hipGraphLaunch(c->g_tail[src], c->stream));
+            queued += (long long)first * M + (long long)extra * Mt;
         } else if (P > 0) {
             for (int k = 0; k < tails; ++k) HIPC(c, hipGraphLaunch(c->g_tail[src], c->stream));
             queued += (long long)tails * Mt;
