@@ -374,6 +374,9 @@ def main():
                          "calibration copy (for counter scaling); prints no JSON")
     ap.add_argument("--profile-form", choices=("fp64", "refined"), default="fp64",
                     help="profiling mode: the FP64 step's SOR kernel, or the refined step's FP32 one")
+    ap.add_argument("--profile-adjoint", type=int, default=0, metavar="STEPS",
+                    help="profiling mode: the adjoint workload's forward run (256^2, solves to the "
+                         "relative rule) for STEPS steps; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the single cavity would shard "
@@ -386,6 +389,19 @@ def main():
     dev = cnv.device_name(0)
     print("bench: %s, %dx%d Re=%g, %d warmup + %d timed steps" % (dev, N, N, RE, args.warmup, args.steps),
           file=sys.stderr, flush=True)
+
+    if args.profile_adjoint:
+        n = 256
+        p = cnv.default_params(n, Re=400.0, dt=20 * 2.0 ** -16, beta=cnv.reference_beta(n), itmax=50_000_000,
+                               diagnostics=0)
+        with cnv.Adjoint(p, checkpoint_every=100, rtol_forward=1e-14, rtol_adjoint=1e-14) as adj:
+            t0 = time.perf_counter()
+            adj.forward(cnv.gaussian_vorticity(n, count=32, seed=1), 1.0, args.profile_adjoint)
+            t = time.perf_counter() - t0
+            fs, _ = adj.sweeps()
+        print("bench: profile mode: adjoint forward %d steps %.3f s, %d sweeps" % (args.profile_adjoint, t, fs),
+              file=sys.stderr, flush=True)
+        return
 
     if args.profile_sor:
         refined = args.profile_form == "refined"
