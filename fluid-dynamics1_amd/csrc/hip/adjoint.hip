@@ -234,7 +234,7 @@ struct cnv_adjoint {
     char err[256] = "";
 };
 
-std::mutex g_aerr_mu;  // a->err is written by both threads of a backward pass
+static std::mutex g_aerr_mu;  // a->err is written by both threads of a backward pass
 
 static int afail(cnv_adjoint* a, int code, const char* fmt, ...) {
     char buf[256];
