@@ -220,6 +220,7 @@ struct cnv_adjoint {
     double* d_seg[2] = {nullptr, nullptr};  // [K][3][cells] each: segment s in d_seg[s & 1]
     hipEvent_t ev_ready[2] = {nullptr, nullptr};  // d_seg[b] recomputed (recorded on c's stream)
     hipEvent_t ev_free[2] = {nullptr, nullptr};   // d_seg[b] read by the reverse steps (cz's stream)
+    int seg_stored = -1;  // segment whose step states the last forward run left in its buffer (-1: none)
     double *lw = nullptr, *lu = nullptr, *lv = nullptr, *lwt = nullptr, *lub = nullptr, *lvb = nullptr;
     double *wt = nullptr, *src = nullptr, *dU = nullptr;
     double *uT = nullptr, *vT = nullptr;  // the last forward run's final velocities (the seed)
@@ -509,11 +510,17 @@ int cnv_adj_forward(cnv_adjoint* a, const double* w0, double U, int nsteps, doub
     AHIP(a, hipMemcpy(c->v, v0.data(), bytes, hipMemcpyHostToDevice));
     c->bc.u4 = U;
     a->U = U;
-    a->T = nsteps;
+    a->T = 0;  // until this run has finished: a backward call after a failed run is refused
     a->sweeps_fwd = 0;
+    a->seg_stored = -1;
     if (a->warm) AHIP(a, hipMemsetAsync(c->psi[0], 0, bytes, c->stream));  // step 0 starts from 0
+    const int last = nck - 1;  // its step states go straight to its segment buffer: no recompute
     for (int t = 0; t < nsteps; ++t) {
         const bool ck = t % a->K == 0;
+        if (t >= last * a->K) {
+            int rc = save_state(a, a->d_seg[last & 1], t - last * a->K);
+            if (rc) return rc;
+        }
         if (ck) {
             int rc = save_state(a, a->d_ckpt, t / a->K);
             if (rc) return rc;
@@ -535,6 +542,8 @@ int cnv_adj_forward(cnv_adjoint* a, const double* w0, double U, int nsteps, doub
     double s = 0.0;
     for (size_t q = 0; q < a->cells; ++q) s += u0[q] * u0[q] + v0[q] * v0[q];
     if (energy) *energy = 0.5 * c->p.dx * c->p.dy * s;
+    a->T = nsteps;
+    a->seg_stored = last;
     return CNV_OK;
 }
 
@@ -552,8 +561,9 @@ static int recompute(cnv_adjoint* a, int s, int b) {
 }
 
 // The backward pass runs two pipelines at once.  A second thread recomputes the segments, last
-// first, on the context c; this thread runs each segment's reverse steps on the context cz as
-// soon as its recompute is done.  Segment s goes to d_seg[s & 1], so the recompute of segment s - 1
+// first, on the context c (the last one only when a backward pass since the forward run has reused
+// its buffer); this thread runs each segment's reverse steps on the context cz as soon as its
+// recompute is done.  Segment s goes to d_seg[s & 1], so the recompute of segment s - 1
 // overlaps the reverse steps of segment s; the recompute of s - 2 waits until those reverse steps
 // have read the buffer.  The two streams are ordered by events (ev_ready[b] after a recompute,
 // ev_free[b] after its reverse steps), and each thread queues its wait only after the host has seen
@@ -580,6 +590,9 @@ int cnv_adj_backward(cnv_adjoint* a, double* dE_dw0, double* dE_dU) {
     if (a->warm) AHIP(a, hipMemsetAsync(a->zk, 0, sizeof(double) * a->cells, z->stream));  // the last step's z starts from 0
     a->sweeps_adj = a->sweeps_rec = 0;
     const int nseg = (a->T + a->K - 1) / a->K;
+    // the last segment's states, stored by the forward run, hold until this pass reuses its buffer
+    const bool have_last = a->seg_stored == nseg - 1;
+    a->seg_stored = -1;
 
     std::mutex m;
     std::condition_variable cv;
@@ -618,7 +631,7 @@ int cnv_adj_backward(cnv_adjoint* a, double* dE_dw0, double* dE_dU) {
                     break;
                 }
             }
-            rc = recompute(a, sg, b);
+            if (!(have_last && sg == nseg - 1)) rc = recompute(a, sg, b);
             if (!rc && hipEventRecord(a->ev_ready[b], c->stream) != hipSuccess)
                 rc = afail(a, CNV_ERR_HIP, "hipEventRecord failed in the recompute");
             if (!rc) post(recomputed, sg);

@@ -123,6 +123,19 @@ def test_gpu_adjoint_backward_failure_ends_both_pipelines():
             assert "adjoint-problem Poisson solve did not converge" in str(ei.value)
 
 
+def test_gpu_adjoint_backward_refused_after_failed_forward():
+    # a forward run whose solve hits itmax leaves no trajectory to differentiate
+    n = 24
+    p = cnv.default_params(n, Re=300.0, dt=4e-3, beta=cnv.reference_beta(n), itmax=60, diagnostics=0)
+    with cnv.Adjoint(p, checkpoint_every=2, rtol_forward=1e-14, rtol_adjoint=1e-14) as adj:
+        with pytest.raises(cnv.CnvError) as ei:
+            adj.forward(cnv.gaussian_vorticity(n, count=6, seed=3), 1.0, 6)
+        assert ei.value.code == cnv.NOT_CONVERGED
+        with pytest.raises(cnv.CnvError) as ei:
+            adj.backward()
+        assert ei.value.code == cnv.ERR_ARG and "run cnv_adj_forward first" in str(ei.value)
+
+
 def test_gpu_adjoint_through_torch_autograd():
     torch = pytest.importorskip("torch")
     n, steps, Re, dt = 16, 4, 300.0, 4e-3
