@@ -248,10 +248,11 @@ const char *cnv_adj_last_error(const cnv_adjoint *a);   /* a may be NULL: this t
 /* Forward run from (w0, U) for nsteps; stores the checkpoints; returns E. */
 int cnv_adj_forward(cnv_adjoint *a, const double *w0, double U, int nsteps, double *energy);
 /* Gradient of the last forward run's E: dE_dw0 (host, n*n) and dE_dU (either may be NULL); repeated
- * calls after one forward run return the same gradient.  A second host thread recomputes the
- * segments on a stream of its own while this thread runs the reverse steps of the segment before
- * (two segment buffers of checkpoint_every x 3 planes); the result is the same as running them
- * one after the other.  Returns when both are done. */
+ * calls after one forward run return the same gradient; refused after a forward run that failed.
+ * A second host thread recomputes the segments on a stream of its own while this thread runs the
+ * reverse steps of the segment after (two segment buffers of checkpoint_every x 3 planes; the
+ * forward run leaves its last segment's states in one of them); the result is the same as running
+ * them one after the other.  Returns when both are done. */
 int cnv_adj_backward(cnv_adjoint *a, double *dE_dw0, double *dE_dU);
 /* SOR sweeps of the last forward run and of the last backward run's adjoint solves (its
  * recompute not counted). */
