@@ -619,7 +619,7 @@ int cnv_adj_backward(cnv_adjoint* a, double* dE_dw0, double* dE_dU) {
     };
     int rc_rec = CNV_OK;
     char err_rec[256] = "";
-    std::thread rec([&]() {
+    auto recompute_all = [&]() {
         int rc = CNV_OK;
         if (hipSetDevice(c->p.device) != hipSuccess) rc = afail(a, CNV_ERR_HIP, "hipSetDevice failed in the recompute");
         for (int sg = nseg - 1; sg >= 0 && !rc; --sg) {
@@ -644,7 +644,13 @@ int cnv_adj_backward(cnv_adjoint* a, double* dE_dw0, double* dE_dU) {
             rc_rec = rc;
             halt();
         }
-    });
+    };
+    std::thread rec;
+    try {
+        rec = std::thread(recompute_all);
+    } catch (...) {
+        return afail(a, CNV_ERR_NOMEM, "adjoint backward: could not start the recompute thread");
+    }
     int rc = CNV_OK;
     for (int sg = nseg - 1; sg >= 0 && !rc; --sg) {
         const int b = sg & 1;
