@@ -47,15 +47,15 @@ CLOCK_HZ = 2.4e9              # MI355X peak engine clock
 SIMDS_PER_CU = 4
 
 
-def workload_params(cnv, precision, args, **kw):
-    # FP64 steps replay captured HIP graphs unless --eager; the refined form runs eager launches,
-    # as fast as replay at omega 1.95 and 3% faster with the reference's inner relaxation: its
-    # step needs tens of graph launches (each outer iteration's residual, chunks and correction),
-    # each with its own submission cost
+def workload_params(cnv, precision, args, graph=True, **kw):
+    # steps replay captured HIP graphs unless --eager (or graph=False: the refined form with the
+    # reference's inner relaxation runs eager launches, 4% faster there, since its short inner
+    # solves leave each outer iteration's residual, chunks and correction graph launches with
+    # their own submission cost; at omega 1.95 replay is 1% faster)
     return cnv.default_params(N, Re=RE, dt=DT, order=6, beta=OMEGA, tol=TOL, itmax=50_000_000,
                               precision=precision, sweeps_per_launch=args.spl, tile_waves=args.tile_waves,
                               wave_rows=args.wave_rows, diagnostics=1,
-                              use_graph=0 if args.eager else int(precision == cnv.PREC_FP64), **kw)
+                              use_graph=0 if args.eager or not graph else 1, **kw)
 
 
 def timed_steps(cav, k, tag):
@@ -307,7 +307,7 @@ def adjoint_leg(cnv, steps, K):
 
 def dc_leg(cnv, args, state64, ms64, spl, inner_beta, tag, time_sor=True, transport_fp32=1):
     """The FP32 + FP64 defect-correction form of the same steps, compared with the FP64 run."""
-    p = workload_params(cnv, cnv.PREC_FP32_DC, args)
+    p = workload_params(cnv, cnv.PREC_FP32_DC, args, graph=inner_beta <= 0 and not args.refined_eager)
     p.inner_beta = inner_beta
     p.transport_fp32 = transport_fp32
     cd = cnv.Cavity(p)
@@ -327,6 +327,7 @@ def dc_leg(cnv, args, state64, ms64, spl, inner_beta, tag, time_sor=True, transp
 
     out = {
         "inner_beta": inner_beta if inner_beta > 0 else OMEGA,
+        "step": "captured HIP graphs" if p.use_graph else "eager launches",
         "stencils": "fp32" if transport_fp32 else "fp64",  # transport and velocity stencils
         "value": C_CELLS * args.steps / (ms32 / 1e3) / 1e6,
         "ms_per_step": ms32 / args.steps,
@@ -374,6 +375,10 @@ def main():
                          "calibration copy (for counter scaling); prints no JSON")
     ap.add_argument("--profile-form", choices=("fp64", "refined"), default="fp64",
                     help="profiling mode: the FP64 step's SOR kernel, or the refined step's FP32 one")
+    ap.add_argument("--refined-eager", action="store_true",
+                    help="run the refined legs with eager launches (for rocprofv3 --kernel-trace, whose "
+                         "interception of graph launches has faulted when a second context replayed its "
+                         "graphs in one process)")
     ap.add_argument("--profile-adjoint", type=int, default=0, metavar="STEPS",
                     help="profiling mode: the adjoint workload's forward run (256^2, solves to the "
                          "relative rule) for STEPS steps; prints no JSON")
@@ -405,7 +410,8 @@ def main():
 
     if args.profile_sor:
         refined = args.profile_form == "refined"
-        cav = cnv.Cavity(workload_params(cnv, cnv.PREC_FP32_DC if refined else cnv.PREC_FP64, args))
+        cav = cnv.Cavity(workload_params(cnv, cnv.PREC_FP32_DC if refined else cnv.PREC_FP64, args,
+                                         graph=not (refined and args.refined_eager)))
         timed_steps(cav, 1, "dc" if refined else "fp64")
         ms = cav.time_sor(args.profile_sor)
         cms = cav.calibration_copy(256 << 20)

@@ -29,7 +29,7 @@ class Params(C.Structure):
         ("sweeps_per_launch", C.c_int), ("tile_waves", C.c_int), ("wave_rows", C.c_int),
         ("launches_per_poll", C.c_int), ("diagnostics", C.c_int), ("use_graph", C.c_int),
         ("inner_beta", C.c_double), ("warm_start", C.c_int), ("transport_fp32", C.c_int),
-        ("blowup_speed", C.c_double),
+        ("blowup_speed", C.c_double), ("inner_test_every", C.c_int),
     ]
 
 

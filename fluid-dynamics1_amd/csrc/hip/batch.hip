@@ -383,6 +383,9 @@ static int build_sor(cnv_batch* b, TileShape sh) {
         }
         if (b->dc) {
             SorArgs<float> a = sor_args32(c);
+            // by default every launch tests its last sweep: the batch's outer iterations run in
+            // lockstep, so a late test on any cavity's short inner solve holds up every cavity
+            a.test_mask = inner_test_every(c->p, 1) - 1;
             a.ctl = b->d_ctl + 2 * i;
             a.partial = b->d_partial + b->part_off[i];
             a.tiles_i = ti[i];

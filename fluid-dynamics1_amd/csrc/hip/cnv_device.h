@@ -48,6 +48,9 @@ template <typename T>
 struct SorPlanes {
     static constexpr int value = sizeof(T) == 8 ? 3 : 2;
 };
+// Launches per stopping test of a one-cavity context's FP32 inner solves (SorArgs::test_mask + 1):
+// only the launches with L % 4 == 3 test their last sweep; a batch tests every launch.
+constexpr int kFp32TestEvery = 4;
 
 // The planes of an FP64 solve, for the kernels that read its final iterate (plane fin->parity).
 struct Planes {
@@ -107,6 +110,11 @@ struct SorArgs {
     T rB;                // RN(1/B)
     int exact_rcp;       // 1: 1/B is exact (B a power of two), so x*rB == RN(x/B)
     int norm;            // FP64: also sum the iterate's L1 norm (needed when rtol > 0)
+    int test_mask;       // FP32: the stopping rule is tested on the last sweep of the launches
+                         // with (launch & test_mask) == test_mask only (test_mask + 1, a power
+                         // of two, launches per test); the others sum nothing, leave no
+                         // partials, and their decision carries the state forward.  FP64
+                         // tests every sweep (0)
 };
 
 // Batched SOR launch: workgroup -> (solve, tile); inst < 0 marks padding workgroups, tile ==

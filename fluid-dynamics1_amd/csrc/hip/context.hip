@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <numeric>
 #include <vector>
 
 #include "cnv.h"
@@ -309,6 +310,9 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
     if (p->precision != CNV_PREC_FP64 && p->precision != CNV_PREC_FP32_DC)
         return cnv_fail(nullptr, CNV_ERR_ARG, "invalid precision %d", p->precision);
     if (!(p->dx > 0) || !(p->dy > 0)) return cnv_fail(nullptr, CNV_ERR_ARG, "grid spacing must be positive");
+    if (p->inner_test_every < 0 || p->inner_test_every > 64 || (p->inner_test_every & (p->inner_test_every - 1)))
+        return cnv_fail(nullptr, CNV_ERR_ARG, "inner_test_every=%d: need 0 (default) or a power of two up to 64",
+                        p->inner_test_every);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p->device || p->device < 0)
         return cnv_fail(nullptr, CNV_ERR_NO_DEVICE, "HIP device %d not available (%d devices)", p->device, ndev);
@@ -340,6 +344,7 @@ int cnv_create(const cnv_params* p, cnv_ctx** out) {
     // measured 0.6% faster per step than 32 or 128 (the chunk length is a multiple of 6, see
     // build_graphs).  Eager solves choose their chunk per solve (run_sor).
     c->launches_per_poll = p->launches_per_poll > 0 ? p->launches_per_poll : 66;
+    c->tail_chunk = p->precision == CNV_PREC_FP32_DC ? std::lcm(6, inner_test_every(*p, kFp32TestEvery)) : 6;
     c->poll_fixed = p->launches_per_poll > 0 ? p->launches_per_poll : 0;
     c->tiles_i = (c->n + tile_rows(c->shape) - 1) / tile_rows(c->shape);
     c->tiles_j = (c->n + tile_cols(c->shape) - 1) / tile_cols(c->shape);
@@ -429,12 +434,13 @@ static int poisson_dc(cnv_ctx* c, const double* w_src, SorCtl* fin, int* outer) 
 }
 
 // ---- captured step (use_graph) -------------------------------------------------------------
-// Launch index L only matters through L mod 2 (state slots, partial sums) and L mod 3 (FP64
-// iterate planes), and every solve's state lives in the device slots, so a captured chunk of M
-// (a multiple of 6) SOR launches with L = 0..M-1 is replayed until the solve ends: each replay
-// continues the previous one's rotation, and after whole chunks the final slot is 1.  Two chunk
-// sizes: launches_per_poll for the predicted length of the solve and kTailChunk for its end, so
-// that the no-op launches queued after convergence stay few.
+// Launch index L only matters through L mod 2 (state slots, partial sums), L mod 3 (FP64
+// iterate planes) and L mod 4 (the FP32 inner solves' stopping tests), and every solve's state
+// lives in the device slots, so a captured chunk of M (a multiple of tail_chunk: 6 FP64, 12
+// FP32) SOR launches with L = 0..M-1 is replayed until the solve ends: each replay continues the
+// previous one's rotation, and after whole chunks the final slot is 1.  Two chunk sizes:
+// launches_per_poll for the predicted length of the solve and tail_chunk for its end, so that
+// the no-op launches queued after convergence stay few.
 static int capture(cnv_ctx* c, hipGraphExec_t* out, int (*body)(cnv_ctx*, int), int arg) {
     hipGraph_t g = nullptr;
     HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -496,7 +502,7 @@ static int body_chunk_of(cnv_ctx* c, int src, int m) {
     return CNV_OK;
 }
 static int body_chunk(cnv_ctx* c, int src) { return body_chunk_of(c, src, c->launches_per_poll); }
-static int body_tail(cnv_ctx* c, int src) { return body_chunk_of(c, src, cnv_ctx::kTailChunk); }
+static int body_tail(cnv_ctx* c, int src) { return body_chunk_of(c, src, c->tail_chunk); }
 // refined form, one outer iteration: FP64 residual of psi[0] against w[src], inner solve init
 // with the remaining budget (device counter) ...
 static int body_resid(cnv_ctx* c, int src) {
@@ -522,8 +528,9 @@ static int body_post(cnv_ctx* c, int) {
 }
 
 static int build_graphs(cnv_ctx* c) {
-    // whole chunks end on slot 1 and on the plane rotation's phase (launch index mod 2 and mod 3)
-    c->launches_per_poll = (c->launches_per_poll + 5) / 6 * 6;
+    // whole chunks end on slot 1 and on the launch index's phase (plane rotation, stopping tests)
+    const int u = c->tail_chunk;
+    c->launches_per_poll = (c->launches_per_poll + u - 1) / u * u;
     const int nsrc = is_dc(c) ? 1 : 2;  // the refined form's inner solves always read r32
     for (int wc = 0; wc < 2; ++wc) {
         int rc = capture(c, &c->g_pre[wc], body_pre, wc);
@@ -540,11 +547,11 @@ static int build_graphs(cnv_ctx* c) {
 }
 
 // Replays chunk graphs until a polled slot is terminal.  With a prediction of P launches, big
-// chunks and then tail chunks up to about P - 2*kTailChunk launches go before the first poll, and
+// chunks and then tail chunks up to about P - 2*tail_chunk launches go before the first poll, and
 // tail chunks follow, as many per poll as make up the eager solve's chunk for that prediction
 // (poll_chunk); without one, big chunks one per poll.
 static int replay_solve(cnv_ctx* c, int src, int predicted_sweeps, SorCtl* final_ctl) {
-    const int M = c->launches_per_poll, Mt = cnv_ctx::kTailChunk;
+    const int M = c->launches_per_poll, Mt = c->tail_chunk;
     const int S = c->shape.s;
     const int P = predicted_sweeps > 0 ? predicted_sweeps / S : 0;
     const long long cap = (long long)c->p.itmax / S + 4LL * M + 8;

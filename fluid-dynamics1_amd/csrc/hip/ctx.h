@@ -43,17 +43,19 @@ struct cnv_ctx {
     // launch shape
     cnv::TileShape shape{5, 16, 4};
     int tiles_i = 0, tiles_j = 0;
-    int launches_per_poll = 66;  // SOR launches per captured chunk (graph path), a multiple of 6
+    int launches_per_poll = 66;  // SOR launches per captured chunk (graph path), a multiple of tail_chunk
     int poll_fixed = 0;   // launches per poll as requested; 0: chosen per solve (eager path)
     int k_prev = 0;       // sweeps the next solve is predicted to take (the previous one's)
     static constexpr int kDcPred = 64;
     int k_dc[kDcPred] = {};  // FP32_DC: sweeps of the inner solve of each outer iteration
     cnv::WallBC bc{};
     // captured step (use_graph): pre-solve launches per w parity, a chunk of launches_per_poll
-    // SOR launches and a tail chunk of kTailChunk per source plane (FP64; the refined form's
+    // SOR launches and a tail chunk of tail_chunk per source plane (FP64; the refined form's
     // inner solves use index 0), post-solve launches; the refined form adds, per outer
-    // iteration, a residual + inner-init graph per w parity and a correction graph
-    static constexpr int kTailChunk = 6;
+    // iteration, a residual + inner-init graph per w parity and a correction graph.  Chunk
+    // lengths are multiples of the launch index's period: 6 for FP64 (state slots mod 2, planes
+    // mod 3), 12 for the FP32 inner solves (slots and planes mod 2, stopping tests mod 4).
+    int tail_chunk = 6;
     hipGraphExec_t g_pre[2] = {nullptr, nullptr};
     hipGraphExec_t g_chunk[2] = {nullptr, nullptr};
     hipGraphExec_t g_tail[2] = {nullptr, nullptr};
@@ -116,6 +118,7 @@ inline cnv::SorArgs<T> sor_args_for(int n, int ld, double dx, double dy, double 
     int e2 = 0;
     const double Bd = 2 * (dx * dx + dy * dy);
     a.exact_rcp = std::frexp(Bd, &e2) == 0.5 && std::isnormal(1.0 / Bd) && (1.0 / Bd) * Bd == 1.0;
+    a.test_mask = 0;
     return a;
 }
 
@@ -124,12 +127,16 @@ inline cnv::SorArgs<double> sor_args64(cnv_ctx* c, const double* src, int negate
     return sor_args_for<double>(c->n, c->ld, c->p.dx, c->p.dy, c->p.beta, c->tiles_i, c->tiles_j, c->partial, c->ctl,
                                 src, c->psi[0], c->psi[1], c->psi[2], negate);
 }
+// Launches per stopping test of FP32 inner solves: the parameter, or the given default.
+inline int inner_test_every(const cnv_params& p, int dflt) { return p.inner_test_every > 0 ? p.inner_test_every : dflt; }
 // The context's FP32 inner solve of defect correction: source r32, planes e32[0..1]; the inner
-// solves may relax with their own factor.
+// solves may relax with their own factor and test their stopping rule every few launches.
 inline cnv::SorArgs<float> sor_args32(cnv_ctx* c) {
     const double beta = c->p.inner_beta > 0 ? c->p.inner_beta : c->p.beta;
-    return sor_args_for<float>(c->n, c->ld, c->p.dx, c->p.dy, beta, c->tiles_i, c->tiles_j, c->partial, c->ctl,
-                               c->r32, c->e32[0], c->e32[1], nullptr, 0);
+    cnv::SorArgs<float> a = sor_args_for<float>(c->n, c->ld, c->p.dx, c->p.dy, beta, c->tiles_i, c->tiles_j,
+                                                c->partial, c->ctl, c->r32, c->e32[0], c->e32[1], nullptr, 0);
+    a.test_mask = inner_test_every(c->p, cnv::kFp32TestEvery) - 1;
+    return a;
 }
 // The stencils of the refined form's transport and velocities run in FP32 (transport_fp32).
 inline bool fp32_stencils(const cnv_ctx* c) {

@@ -63,11 +63,12 @@ typedef struct {
     int wave_rows;         /* grid rows held in registers per wave (compiled shapes only)    */
     int launches_per_poll; /* SOR launches queued between host polls of the convergence flag
                               (0: per solve from the previous solve's length, 66 per captured
-                              chunk with use_graph, 2 for a batch)                          */
+                              chunk with use_graph (72 for FP32_DC: chunks hold multiples of 6,
+                              12 for FP32_DC), 2 for a batch)                               */
     int diagnostics;       /* 1 = continuity max/min every step (src/main.c:387-408)        */
     int use_graph;         /* 1 = each step replays captured HIP graphs (the pre-solve
                               launches, chunks of launches_per_poll SOR launches and tail
-                              chunks of 6, the post-solve launches; FP32_DC also the residual
+                              chunks of 6 (12 for FP32_DC), the post-solve launches; FP32_DC also the residual
                               and correction of each outer iteration, with the sweep budget
                               kept on the device); results identical to the eager launches   */
     double inner_beta;     /* FP32_DC: relaxation of the inner FP32 SOR solves (0 = beta)     */
@@ -88,6 +89,13 @@ typedef struct {
                               ends with CNV_DIVERGED (the explicit scheme has gone unstable);
                               0 = off, as in the reference.  In a batch the cavity is dropped
                               from later steps and the others go on                           */
+    int inner_test_every;  /* FP32_DC: the inner FP32 solves test their stopping rule on the last
+                              sweep of every k-th SOR launch only (the others skip the sums and
+                              their decision); k a power of two up to 64.  0 = 4 for a context (6.1 against 6.4 us per
+                              launch at 1024^2), 1 for a batch, whose outer iterations run in
+                              lockstep so that a late test on any cavity's short inner solve
+                              holds up all of them.  A solve may run up to k*S - 1 sweeps past
+                              the first sweep that met the rule                               */
 } cnv_params;
 
 typedef struct {

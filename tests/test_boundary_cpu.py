@@ -226,6 +226,10 @@ def test_device_api_argument_errors_without_gpu():
     assert L.cnv_sweeps_per_launch(None) == 0
     assert L.cnv_batch_create(ctypes.byref(p), 1, None) == cnv.ERR_ARG
     assert L.cnv_adj_create(ctypes.byref(p), 10, 1e-12, 1e-12, None) == cnv.ERR_ARG
+    for every in (-1, 3, 128):
+        bad_every = cnv.default_params(32, inner_test_every=every)
+        assert L.cnv_create(ctypes.byref(bad_every), ctypes.byref(h)) == cnv.ERR_ARG and not h.value
+        assert b"inner_test_every" in L.cnv_last_error(None)
     # the fallback message (no context) belongs to the calling thread
     assert L.cnv_step(None, -1, None) == cnv.ERR_ARG
     mine = L.cnv_last_error(None)

@@ -37,11 +37,14 @@ def run_alone(p, steps):
 
 
 @pytest.mark.parametrize("shape", [(5, 16, 4), (5, 4, 6)], ids=["s5_16x4", "s5_4x6"])
-@pytest.mark.parametrize("precision,steps", [(cnv.PREC_FP64, 3), (cnv.PREC_FP32_DC, 2)])
-def test_batch_equals_single(precision, steps, shape):
+@pytest.mark.parametrize("precision,steps,every", [(cnv.PREC_FP64, 3, 0), (cnv.PREC_FP32_DC, 2, 1),
+                                                   (cnv.PREC_FP32_DC, 2, 4)], ids=["fp64", "refined1", "refined4"])
+def test_batch_equals_single(precision, steps, every, shape):
     # the same launch shape on both sides, named explicitly (a one-cavity default follows the
-    # cavity's size): the reported L1 update is summed per tile
-    ps = mix_params(precision, sweeps_per_launch=shape[0], tile_waves=shape[1], wave_rows=shape[2])
+    # cavity's size): the reported L1 update is summed per tile; and the same stopping-test
+    # cadence of the FP32 inner solves (the defaults differ: 4 for a context, 1 for a batch)
+    ps = mix_params(precision, sweeps_per_launch=shape[0], tile_waves=shape[1], wave_rows=shape[2],
+                    inner_test_every=every)
     print("batch of", [p.n for p in ps], "precision", precision, "shape", shape, flush=True)
     check_batch_equals_single(ps, steps)
 
@@ -49,7 +52,8 @@ def test_batch_equals_single(precision, steps, shape):
 def test_batch_fp64_stencils_equals_single():
     # the refined form with transport_fp32 = 0: the batched transport and velocity kernels' FP64
     # instantiations (the other batch tests run the default FP32 stencils)
-    ps = mix_params(cnv.PREC_FP32_DC, transport_fp32=0, sweeps_per_launch=5, tile_waves=16, wave_rows=4)
+    ps = mix_params(cnv.PREC_FP32_DC, transport_fp32=0, sweeps_per_launch=5, tile_waves=16, wave_rows=4,
+                    inner_test_every=1)
     print("batch of", [p.n for p in ps], "refined, FP64 stencils", flush=True)
     check_batch_equals_single(ps, 2)
 

@@ -12,6 +12,14 @@ pytestmark = pytest.mark.gpu
 
 SHAPE = dict(sweeps_per_launch=5, tile_waves=4, wave_rows=6)
 S = 5
+# launches a non-finite update can take to be seen: the launch that made it, decided by the next
+# one; a one-cavity context's FP32 inner solves (refined form) test the last sweep of every fourth
+# launch only (launches 3, 7, ...), so theirs is seen after the fourth launch; a batch tests every
+# launch
+
+
+def max_sweeps_to_diverged(precision, batch=False):
+    return (4 if precision == cnv.PREC_FP32_DC and not batch else 2) * S
 
 
 def params(n, Re, U, dt, precision=cnv.PREC_FP64, **kw):
@@ -25,7 +33,7 @@ MIX = [(32, 400.0, 1.0, 2e-4), (48, 1000.0, 0.8, 1e-4), (40, 250.0, 1.2, 3e-4), 
 
 @pytest.mark.parametrize("precision", [cnv.PREC_FP64, cnv.PREC_FP32_DC], ids=["fp64", "refined"])
 @pytest.mark.parametrize("graph", [0, 1], ids=["eager", "graph"])
-def test_nan_vorticity_ends_the_step_within_two_launches(precision, graph):
+def test_nan_vorticity_ends_the_step_within_a_few_launches(precision, graph):
     p = params(48, 1000.0, 1.0, 1e-4, precision=precision, use_graph=graph)
     print("one cavity, NaN vorticity, precision", precision, "graph", graph, flush=True)
     with cnv.Cavity(p) as cav:
@@ -37,7 +45,7 @@ def test_nan_vorticity_ends_the_step_within_two_launches(precision, graph):
         with pytest.raises(cnv.CnvError) as ei:
             cav.step(1)
     assert st.status == cnv.DIVERGED
-    assert 0 < st.sweeps <= 2 * S  # the first launch's sweeps, decided by the second launch
+    assert 0 < st.sweeps <= max_sweeps_to_diverged(precision)
     assert ei.value.code == cnv.DIVERGED
 
 
@@ -71,7 +79,7 @@ def test_batch_nan_cavity_drops_out_others_unchanged(precision):
     st, status, got = run_batch(ps, 4, inject=1, at=1)
     assert ref_status == [cnv.OK] * 4
     assert status == [cnv.OK, cnv.DIVERGED, cnv.OK, cnv.OK]
-    assert st[1][1].status == cnv.DIVERGED and 0 < st[1][1].sweeps <= 2 * S
+    assert st[1][1].status == cnv.DIVERGED and 0 < st[1][1].sweeps <= max_sweeps_to_diverged(precision, batch=True)
     for t in (2, 3):  # left out of the later steps
         assert st[t][1].status == cnv.DIVERGED and st[t][1].sweeps == 0
     for i in (0, 2, 3):
