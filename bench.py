@@ -51,7 +51,7 @@ def workload_params(cnv, precision, args, graph=True, **kw):
     # steps replay captured HIP graphs unless --eager (or graph=False: the refined form with the
     # reference's inner relaxation runs eager launches, 4% faster there, since its short inner
     # solves leave each outer iteration's residual, chunks and correction graph launches with
-    # their own submission cost; at omega 1.95 replay is 1% faster)
+    # their own submission cost; at omega 1.95 the two are within 1% of each other)
     return cnv.default_params(N, Re=RE, dt=DT, order=6, beta=OMEGA, tol=TOL, itmax=50_000_000,
                               precision=precision, sweeps_per_launch=args.spl, tile_waves=args.tile_waves,
                               wave_rows=args.wave_rows, diagnostics=1,
