@@ -43,8 +43,10 @@
 //   * if sweep s < S-1 of launch L reached e < tol, launch L's output is discarded too: launch
 //     L+2 redoes s+1 sweeps from launch L's input plane, which FP64 keeps intact by rotating
 //     through three planes, so the final iterate is exactly the one of the first sweep with
-//     e < tol (src/poisson.c:273); FP32 planes, a preconditioner, test the last sweep of each
-//     launch only and rotate through two;
+//     e < tol (src/poisson.c:273); FP32 planes, a preconditioner, test the last sweep of a
+//     launch only, and only of the launches with (L & test_mask) == test_mask (the others sum
+//     nothing, leave no partials, and their decider carries the state forward), and rotate
+//     through two;
 //   * the sweep budget itmax is honoured exactly (src/poisson.c:234);
 //   * a non-finite L1 update (or iterate norm) ends the solve with SOR_DIVERGED;
 //   * relative rule: with rtol > 0 the solve also stops ("fold") once the update norm of a sweep
