@@ -190,9 +190,10 @@ __device__ __forceinline__ void sor_rb_sweeper(const SorArgs<T>& a, const int L,
     // planes, on request (NORM) for FP64.  It is measured once per launch, after its last sweep.
     constexpr bool kNorm = sizeof(T) == 4 || NORM;
     // FP32 planes (the inner solves of defect correction) are a preconditioner with no bit-parity
-    // requirement: their stopping rule is applied to each launch's last sweep only, so the other
-    // sweeps skip the L1 update (a solve runs at most S-1 sweeps past the first sweep that met
-    // the rule).  FP64 applies it to every sweep, as src/poisson.c:272-278 does.
+    // requirement: their stopping rule is applied to a launch's last sweep only (and only on the
+    // launches the test mask selects), so the other sweeps skip the L1 update (a solve runs at
+    // most (test_mask + 1) * S - 1 sweeps past the first sweep that met the rule).  FP64 applies
+    // it to every sweep, as src/poisson.c:272-278 does.
     constexpr bool kLastOnly = sizeof(T) == 4;
     constexpr int SE = kLastOnly ? 1 : S;  // L1 update sums per launch
     constexpr int EJ = 128;          // extended tile columns; lane m holds columns 2m, 2m+1

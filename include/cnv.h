@@ -54,7 +54,8 @@ typedef struct {
     double v1, v2, v3, v4;
     /* FP32_DC only */
     double inner_rtol;     /* inner FP32 solve: fold once its L1 update < inner_rtol * L1 norm of
-                              the inner iterate, tested on the last sweep of each launch      */
+                              the inner iterate, tested on the last sweep of a launch (every
+                              inner_test_every-th launch)                                     */
     int max_outer;         /* defect-correction iterations allowed per solve                */
     /* tuning (0 = automatic) */
     int sweeps_per_launch; /* SOR sweeps fused per kernel launch (compiled: 1, 2, 4, 5, 6, 8; 0 =
